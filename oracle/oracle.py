@@ -1,0 +1,306 @@
+"""Python side of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker or as the timed CPU
+baseline.  The product (``cupoly_amd``) never imports it.
+
+It wraps ``oracle/liboracle.so`` (built from ``cupoly_oracle.c`` by
+``oracle/Makefile``) with numpy, and computes the engine's RNS constants with
+exact Python integers, restating ``CUDAEngine::gen_rns_primes`` and
+``CUDAEngine::precompute`` (``src/cuda/cudaengine.cu:1249-1511``) and the DGT
+root choice of ``DGTEngine::init`` (``src/cuda/dgt.cu:1694-1982``).
+
+Parity status of each oracle function is documented in the C file's header
+and in the tests that pin it.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dblp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+_lib = None
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        u64, i32, dbl = ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+        L.or_mulmod.restype = u64
+        L.or_mulmod.argtypes = [u64, u64, u64]
+        L.or_powmod.restype = u64
+        L.or_powmod.argtypes = [u64, u64, u64]
+        L.or_dgt_forward.argtypes = [_u64p, i32, u64, _u64p, u64]
+        L.or_dgt_inverse.argtypes = [_u64p, i32, u64, _u64p, u64]
+        L.or_dgt_poly.argtypes = [_u64p, i32, i32, _u64p, _u64p, _u64p, i32]
+        L.or_pointwise.argtypes = [_u64p, _u64p, _u64p, _u64p, i32, i32, _u64p, i32]
+        L.or_polymul_batch.argtypes = [_u64p, _u64p, _u64p, i32, i32, i32, _u64p,
+                                       _u64p, _u64p, _u64p]
+        L.or_negacyclic_schoolbook.argtypes = [_u64p, _u64p, _u64p, i32, u64]
+        L.or_fma_rd.restype = dbl
+        L.or_fma_rd.argtypes = [dbl, dbl, dbl]
+        L.or_rcp_rd.restype = dbl
+        L.or_rcp_rd.argtypes = [u64]
+        L.or_basis_ext_q_to_b.argtypes = [_u64p, _u64p, i32, i32, i32, _u64p, _u64p,
+                                          _u64p, _u64p, _u64p, _dblp, _u64p]
+        L.or_complex_scaling.argtypes = [_u64p, _u64p, _u64p, i32, i32, i32, _u64p,
+                                         _u64p, _u64p, _u64p]
+        L.or_basis_ext_b_to_q.argtypes = [_u64p, _u64p, i32, i32, i32, _u64p, _u64p,
+                                          _u64p, _u64p, _u64p, _u64p]
+        L.or_simple_scaling.argtypes = [_u64p, _u64p, i32, i32, u64, _u64p, _u64p]
+        L.or_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.or_sample_uniform_poly.argtypes = [_u64p, i32, i32, _u64p, u64, u64]
+        L.or_sample_small_poly.argtypes = [_i64p, i32, i32, u64, u64, _u64p, i32]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_u64p):
+    return a.ctypes.data_as(t)
+
+
+def u64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+# --------------------------------------------------------------------------
+# Tables and parameters
+# --------------------------------------------------------------------------
+_tables = None
+_cdt = None
+
+
+def tables():
+    global _tables
+    if _tables is None:
+        with open(os.path.join(REPO, "tests", "golden", "tables63.json")) as f:
+            t = json.load(f)
+        t["primes"] = [int(p) for p in t["primes"]]
+        t["nthroot"] = [[(int(a), int(b)) for a, b in row] for row in t["nthroot"]]
+        t["invnthroot"] = [[(int(a), int(b)) for a, b in row] for row in t["invnthroot"]]
+        _tables = t
+    return _tables
+
+
+def cdt():
+    global _cdt
+    if _cdt is None:
+        with open(os.path.join(REPO, "tests", "golden", "cdt.json")) as f:
+            c = json.load(f)
+        _cdt = (np.array([int(x) for x in c["cdt"]], dtype=np.uint64), c["bound"],
+                [float(x) for x in c["probs"]])
+    return _cdt
+
+
+def prod(xs):
+    r = 1
+    for x in xs:
+        r *= x
+    return r
+
+
+class Params:
+    """RNS parameters for (k, kl, n, t): Q = first k bucket primes, B = next kl
+    (cudaengine.cu:1265-1289), roots for Ng = n/2 (dgt.cu:1526-1538, 1900-1904)."""
+
+    def __init__(self, k, kl, n, t):
+        T = tables()
+        self.k, self.kl, self.n, self.t = k, kl, n, t
+        self.Ng = n // 2
+        si = T["sizes"].index(self.Ng)
+        allp = T["primes"][:k + kl]
+        self.q = allp[:k]
+        self.b = allp[k:]
+        self.primes = allp
+        self.xi = [T["nthroot"][r][si] for r in range(k + kl)]
+        self.xiinv = [T["invnthroot"][r][si] for r in range(k + kl)]
+        self.omega = [pow(T["proot"][r], (p - 1) // self.Ng, p) for r, p in enumerate(allp)]
+        Q, B = prod(self.q), prod(self.b)
+        self.Q, self.B = Q, B
+        q, b = self.q, self.b
+        # cudaengine.cu:1363-1411
+        self.invQqi = [pow((Q // qj) % qj, -1, qj) for qj in q]
+        self.Qqibi = [(Q // qj) % bi for bi in b for qj in q]          # [i][j]
+        self.Qbi = [Q % bi for bi in b]
+        self.invBbi = [pow((B // bi) % bi, -1, bi) for bi in b]
+        self.Bbiqi = [(B // bi) % qj for qj in q for bi in b]          # [j][i]
+        self.Bqi = [B % qj for qj in q]
+        # cudaengine.cu:1463-1511 (exact; the reference uses 512-bit RR)
+        self.lam = [t * pow((B * Q // bi) % bi, -1, bi) * (B // bi) % bi for bi in b]
+        self.wint, self.wfrac = [], []
+        for bi in b:
+            for qj in q:
+                aux = t * pow((Q * B // qj) % qj, -1, qj) * B
+                self.wint.append((aux // qj) % bi)
+                fr = ((aux % qj) << 128) // qj
+                self.wfrac += [fr & (2**64 - 1), fr >> 64]
+        self.twint, self.twfrac = [], []
+        for qj in q:
+            aux = t * pow((Q // qj) % qj, -1, qj)
+            self.twint.append(aux // qj)
+            fr = ((aux % qj) << 128) // qj
+            self.twfrac += [fr & (2**64 - 1), fr >> 64]
+
+    def base_primes(self, base):
+        return {0: self.q, 1: self.b, 2: self.primes}[base]
+
+    def base_roots(self, base):
+        off = {0: 0, 1: self.k, 2: 0}[base]
+        nres = len(self.base_primes(base))
+        return (self.xi[off:off + nres], self.xiinv[off:off + nres],
+                self.omega[off:off + nres])
+
+
+# --------------------------------------------------------------------------
+# numpy wrappers.  A "GI array" is uint64 [..., Ng, 2] (re, imag).
+# --------------------------------------------------------------------------
+def fold(coefs, Ng):
+    """Flat coefficients [..., nres, 2Ng] -> GI [..., nres, Ng, 2]
+    (poly_crt fold, polynomial.cu:206-217)."""
+    c = np.asarray(coefs, dtype=np.uint64)
+    return np.ascontiguousarray(np.stack([c[..., :Ng], c[..., Ng:]], axis=-1))
+
+
+def unfold(gi):
+    g = np.asarray(gi)
+    return np.ascontiguousarray(np.concatenate([g[..., 0], g[..., 1]], axis=-1))
+
+
+def dgt(P, data, base, direction):
+    """DGTEngine::execute_dgt restated: data GI [nres, Ng, 2] (copy returned)."""
+    x = u64(data).copy()
+    pr = P.base_primes(base)
+    xi, xiinv, om = P.base_roots(base)
+    roots = xi if direction == 0 else xiinv
+    lib().or_dgt_poly(_p(x), P.Ng, len(pr), _p(u64(pr)), _p(u64(roots).reshape(-1)),
+                      _p(u64(om)), direction)
+    return x
+
+
+def pointwise(P, a, b, c=None, base=0, op=2):
+    pr = P.base_primes(base)
+    a, b = u64(a), u64(b)
+    out = np.empty_like(a)
+    cc = u64(c) if c is not None else a
+    lib().or_pointwise(_p(out), _p(a), _p(b), _p(cc), P.Ng, len(pr), _p(u64(pr)), op)
+    return out
+
+
+def polymul_batch(P, a, b, base=0):
+    """Coefficient-domain products; a, b GI [batch, nres, Ng, 2]."""
+    pr = P.base_primes(base)
+    xi, xiinv, om = P.base_roots(base)
+    a, b = u64(a), u64(b)
+    c = np.empty_like(a)
+    batch = a.shape[0]
+    lib().or_polymul_batch(_p(c), _p(a), _p(b), batch, P.Ng, len(pr), _p(u64(pr)),
+                           _p(u64(xi).reshape(-1)), _p(u64(xiinv).reshape(-1)),
+                           _p(u64(om)))
+    return c
+
+
+def schoolbook(a, b, p):
+    a, b = u64(a), u64(b)
+    c = np.empty_like(a)
+    lib().or_negacyclic_schoolbook(_p(c), _p(a), _p(b), len(a), p)
+    return c
+
+
+def rcp_rd(q):
+    return lib().or_rcp_rd(q)
+
+
+def fma_rd(a, b, c):
+    return lib().or_fma_rd(a, b, c)
+
+
+def basis_ext_q_to_b(P, x_q, want_v=False):
+    """x_q GI [k, Ng, 2] coefficient domain -> GI [kl, Ng, 2]."""
+    out = np.zeros((P.kl, P.Ng, 2), dtype=np.uint64)
+    v = np.zeros(P.n, dtype=np.uint64)
+    rcp = np.array([rcp_rd(qj) for qj in P.q], dtype=np.float64)
+    lib().or_basis_ext_q_to_b(_p(out), _p(u64(x_q)), P.Ng, P.k, P.kl, _p(u64(P.q)),
+                              _p(u64(P.b)), _p(u64(P.invQqi)), _p(u64(P.Qqibi)),
+                              _p(u64(P.Qbi)), _p(rcp, _dblp), _p(v))
+    return (out, v) if want_v else out
+
+
+def complex_scaling(P, a_q, a_b):
+    """a_q GI [k, Ng, 2], a_b GI [kl, Ng, 2] -> uint64 [kl, 2Ng]."""
+    out = np.zeros((P.kl, P.n), dtype=np.uint64)
+    lib().or_complex_scaling(_p(out), _p(u64(a_q)), _p(u64(a_b)), P.Ng, P.k, P.kl,
+                             _p(u64(P.b)), _p(u64(P.wint)), _p(u64(P.wfrac)),
+                             _p(u64(P.lam)))
+    return out
+
+
+def basis_ext_b_to_q(P, x_b, want_v=False):
+    """x_b uint64 [kl, 2Ng] -> GI [k, Ng, 2]."""
+    out = np.zeros((P.k, P.Ng, 2), dtype=np.uint64)
+    v = np.zeros(P.n, dtype=np.uint64)
+    lib().or_basis_ext_b_to_q(_p(out), _p(u64(x_b)), P.Ng, P.k, P.kl, _p(u64(P.q)),
+                              _p(u64(P.b)), _p(u64(P.invBbi)), _p(u64(P.Bbiqi)),
+                              _p(u64(P.Bqi)), _p(v))
+    return (out, v) if want_v else out
+
+
+def simple_scaling(P, x_q):
+    out = np.zeros((P.Ng, 2), dtype=np.uint64)
+    lib().or_simple_scaling(_p(out), _p(u64(x_q)), P.Ng, P.k, P.t, _p(u64(P.twint)),
+                            _p(u64(P.twfrac)))
+    return out
+
+
+def philox(ctr, key):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().or_philox4x32_10(_p(c, _u32p), _p(k, _u32p), _p(o, _u32p))
+    return [int(x) for x in o]
+
+
+def sample_uniform(primes, n, seed, stream):
+    out = np.zeros((len(primes), n), dtype=np.uint64)
+    lib().or_sample_uniform_poly(_p(out), n, len(primes), _p(u64(primes)), seed, stream)
+    return out
+
+
+def sample_small(n, kind, seed, stream):
+    """kind 2 = ternary, 3 = discrete Gaussian.  Returns int64 [n]."""
+    table, bound, _ = cdt()
+    out = np.zeros(n, dtype=np.int64)
+    lib().or_sample_small_poly(_p(out, _i64p), n, kind, seed, stream, _p(table), bound)
+    return out
+
+
+def small_to_residues(vals, primes):
+    """Signed small coefficients -> flat residues [nres, n]."""
+    v = np.asarray(vals, dtype=np.int64)
+    return np.stack([np.where(v < 0, np.uint64(p) - (-v).astype(np.uint64),
+                              v.astype(np.uint64)) for p in primes])
+
+
+def crt(P, x_q_gi):
+    """GI [k, Ng, 2] residues -> list of n Python ints in [0, Q)."""
+    flat = unfold(x_q_gi)
+    out = []
+    for c in range(P.n):
+        acc = 0
+        for j, qj in enumerate(P.q):
+            Mj = P.Q // qj
+            acc += int(flat[j, c]) * P.invQqi[j] % qj * Mj
+        out.append(acc % P.Q)
+    return out
