@@ -1,0 +1,139 @@
+"""Benchmark: negacyclic poly-muls/s at n = 2^14, k = 4 on one MI355X.
+
+A step is one pass of the hot path over one batch: `--batch` independent
+products c = a * b mod (X^16384 + 1, q_i), i < 4, coefficient-domain uint64
+residues in and out, inputs resident in HBM.  The batched form is one launch
+of the fused forward-DGT / slot-wise product / inverse-DGT kernel per step.
+Also reported on the same GPU:
+  * plain: the reference's one-launch-per-op order (forward a, forward b,
+    mul, inverse per product), same inputs;
+  * roofline: algorithmic bytes of the fused kernel (24*k*n per product:
+    read a and b, write c; twiddles are resident) over its measured average
+    launch time, against 8 TB/s;
+  * cpu_baseline: the CPU oracle (a C restatement of the same algorithm,
+    OpenMP over products x residues) on a bounded sample of the workload.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--k", type=int, default=4)
+    ap.add_argument("--plain-batch", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path has no multi-GPU "
+              "form here); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import cupoly_amd as cp
+
+    torch.cuda.set_device(0)
+    n, k, B = args.n, args.k, args.batch
+    eng = cp.CUDAEngine(k, k + 1, n, 65537)
+    ctx = cp.Context(eng)
+    ng = n // 2
+    # seeded uniform inputs, generated on the device by the Philox sampler
+    a = cp.empty((B, k, ng, 2))
+    b = cp.empty((B, k, ng, 2))
+    for i in range(B):
+        cp.Sampler.sample(ctx, a[i], cp.UNIFORM, 1234, 2 * i, coeff_domain=True)
+        cp.Sampler.sample(ctx, b[i], cp.UNIFORM, 1234, 2 * i + 1, coeff_domain=True)
+    out = cp.empty((B, k, ng, 2))
+    torch.cuda.synchronize()
+    print("bench: inputs ready (batch %d, n %d, k %d)" % (B, n, k), file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        cp.polymul_batch(ctx, out, a, b)
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    evs[0].record()
+    for s in range(args.steps):
+        cp.polymul_batch(ctx, out, a, b)
+        evs[s + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    ms_per_step = wall * 1e3 / args.steps
+    value = B * args.steps / wall
+    avg_launch_ms = sum(launch_ms) / len(launch_ms)
+    bytes_per_launch = 24 * k * n * B
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    print("bench: batched done %.3f ms/step" % ms_per_step, file=sys.stderr, flush=True)
+
+    # plain: one launch per op, same inputs (copies, as the plain path
+    # transforms its operands in place)
+    PB = min(args.plain_batch, B)
+    pa, pb = a[:PB].clone(), b[:PB].clone()
+    pout = cp.empty((PB, k, ng, 2))
+    cp.polymul_plain(ctx, pout, pa.clone(), pb.clone())  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    copies = [(pa.clone(), pb.clone()) for _ in range(reps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for xa, xb in copies:
+        cp.polymul_plain(ctx, pout, xa, xb)
+    torch.cuda.synchronize()
+    plain_rate = PB * reps / (time.perf_counter() - t1)
+    assert torch.equal(pout, out[:PB]), "plain and batched results differ"
+    print("bench: plain done %.0f poly-mul/s" % plain_rate, file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        P = O.Params(k, k + 1, n, 65537)
+        S = min(args.cpu_sample, B)
+        ha, hb = cp.to_host(a[:S]), cp.to_host(b[:S])
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t2 = time.perf_counter()
+        hc = O.polymul_batch(P, ha, hb)
+        cpu_s = time.perf_counter() - t2
+        assert np.array_equal(hc, cp.to_host(out[:S])), "GPU result differs from the CPU oracle"
+        cpu = {"value": S / cpu_s, "unit": "poly-muls/s", "cores": threads, "kind": "port",
+               "sample": "%d products of the same workload (n=%d, k=%d), OpenMP over "
+                         "products x residues, %.1f s" % (S, n, k, cpu_s)}
+
+    line = {
+        "metric": "negacyclic poly-muls/s at N=2^14,k=4 batched vs plain; %HBM roofline; p99 ms",
+        "value": value, "unit": "poly-muls/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "negacyclic poly-mul mod (X^n+1, q_i), coefficient domain in/out, "
+                               "batched fused DGT->product->iDGT, one launch per batch",
+                   "n": n, "k": k, "batch": B, "primes": "63-bit bucket q_0..q_%d" % (k - 1),
+                   "parallelism": "single GPU"},
+        "plain": {"value": plain_rate, "unit": "poly-muls/s", "batch": PB,
+                  "launches_per_product": 4},
+        "speedup_vs_plain": value / plain_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_polymul_fused<13,1>", "avg_launch_ms": avg_launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,342 @@
+"""cupoly_amd - Python host mirror of cuPoly's polynomial hot path on MI355X.
+
+The product is the gfx950 shared library ``cupoly_amd/libcupoly_amd.so`` and
+its C ABI (``include/cupoly_amd.h``).  This module binds that ABI with ctypes
+so Python callers (tests, bench, serving loop) use the reference's vocabulary:
+
+    CUDAEngine(k, kl, n, t)          CUDAEngine::init      (cudaengine.h:109-123)
+    Context(engine[, stream])        Context()             (context.h:36-109)
+    execute_dgt(ctx, data, base, dir) DGTEngine::execute_dgt (dgt.h:141-146)
+    execute_{add,sub,mul_dgt_gi,...} DGTEngine pointwise   (dgt.h:158-281)
+    execute_polynomial_*             CUDAEngine HPS/scalar (cudaengine.h:159-296)
+    Sampler.sample(ctx, out, kind)   Sampler::sample       (sampler.h:77-87)
+
+Device buffers are torch tensors on a ROCm device holding uint64 words in
+torch.int64 storage, shaped [..., nres, n/2, 2] (re, imag).  There is no CPU
+fallback: every call goes to the HIP library, and a missing library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcupoly_amd.so")
+
+QBase, BBase, QBBase, TBase = 0, 1, 2, 3
+FORWARD, INVERSE = 0, 1
+ADD, SUB, MUL, MULMUL, ADDADD = 0, 1, 2, 3, 4
+DISCRETE_GAUSSIAN, UNIFORM, TERNARY = 0, 5, 6
+
+_u64 = ctypes.c_uint64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# name -> argtypes; every symbol declared in include/cupoly_amd.h
+SIGNATURES = {
+    "cupoly_last_error": ([], ctypes.c_char_p),
+    "cupoly_version": ([], ctypes.c_char_p),
+    "cupoly_engine_create": ([_int, _int, _int, _u64, ctypes.POINTER(_vp)], _int),
+    "cupoly_engine_destroy": ([_vp], _int),
+    "cupoly_get_n_residues": ([_vp, _int], _int),
+    "cupoly_engine_info": ([_vp, _vp, _vp, _vp, _vp, _vp], _int),
+    "cupoly_engine_primes": ([_vp, _vp], _int),
+    "cupoly_engine_constant": ([_vp, ctypes.c_char_p, _vp, _int], _int),
+    "cupoly_hps_constant": ([_int, _int, _u64, ctypes.c_char_p, _vp, _int], _int),
+    "cupoly_ctx_create": ([_vp, _vp, ctypes.POINTER(_vp)], _int),
+    "cupoly_ctx_destroy": ([_vp], _int),
+    "cupoly_ctx_stream": ([_vp], _vp),
+    "cupoly_ctx_sync": ([_vp], _int),
+    "cupoly_dgt": ([_vp, _vp, _int, _int], _int),
+    "cupoly_add_dgt": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_sub_dgt": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_mul_dgt_gi": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_mul_dgt_u64": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_muladd_dgt": ([_vp, _vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_double_add_dgt": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_dr2_dgt": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_negate": ([_vp, _vp, _vp, _int], _int),
+    "cupoly_op_by_int": ([_vp, _vp, _vp, _u64, _int, _int], _int),
+    "cupoly_double_op_by_int": ([_vp, _vp, _vp, _u64, _vp, _vp, _u64, _int, _int], _int),
+    "cupoly_basis_ext_q_to_b": ([_vp, _vp], _int),
+    "cupoly_complex_scaling": ([_vp, _vp, _vp, _vp], _int),
+    "cupoly_basis_ext_b_to_q": ([_vp, _vp, _vp], _int),
+    "cupoly_simple_scaling": ([_vp, _vp], _int),
+    "cupoly_poly_basis_ext_q_to_b": ([_vp, _vp, _vp], _int),
+    "cupoly_poly_complex_scaling": ([_vp, _vp, _vp, _vp], _int),
+    "cupoly_sample": ([_vp, _vp, _int, _u64, _u64, _int], _int),
+    "cupoly_sample_coeff": ([_vp, _vp, _int, _u64, _u64, _int], _int),
+    "cupoly_polymul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_polymul_plain": ([_vp, _vp, _vp, _vp, _int], _int),
+}
+
+_lib = None
+
+
+class CupolyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cupoly error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("cupoly_amd: %s missing - run `python -m cupoly_amd.build`"
+                               % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise CupolyError(rc, lib().cupoly_last_error().decode())
+    return rc
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("cupoly_amd: expected a contiguous device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class CUDAEngine:
+    """One ring shape (k, kl, n, t) with its resident tables (CUDAEngine +
+    DGTEngine state).  Several may coexist."""
+
+    def __init__(self, k, kl, n, t):
+        h = ctypes.c_void_p()
+        _check(lib().cupoly_engine_create(k, kl, n, t, ctypes.byref(h)))
+        self.handle = h
+        self.k, self.kl, self.n, self.t = k, kl, n, t
+        self.N = n // 2  # the reference's CUDAEngine::N
+        buf = (ctypes.c_uint64 * (k + kl))()
+        _check(lib().cupoly_engine_primes(h, buf))
+        self.primes = [int(x) for x in buf]
+        self.RNSPrimes = self.primes[:k]
+        self.RNSBPrimes = self.primes[k:]
+
+    def get_n_residues(self, base):
+        return _check(lib().cupoly_get_n_residues(self.handle, base))
+
+    def constant(self, name):
+        buf = (ctypes.c_uint64 * 512)()
+        cnt = _check(lib().cupoly_engine_constant(self.handle, name.encode(), buf, 512))
+        return [int(buf[i]) for i in range(cnt)]
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().cupoly_engine_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def hps_constant(k, kl, t, name):
+    """HPS constants computed on the host alone (no device needed)."""
+    buf = (ctypes.c_uint64 * 512)()
+    cnt = _check(lib().cupoly_hps_constant(k, kl, t, name.encode(), buf, 512))
+    return [int(buf[i]) for i in range(cnt)]
+
+
+class Context:
+    """Stream + scratch.  By default it runs on torch's current stream so that
+    torch reads of the outputs are ordered after the library's kernels."""
+
+    def __init__(self, engine, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        h = ctypes.c_void_p()
+        _check(lib().cupoly_ctx_create(engine.handle, ctypes.c_void_p(handle), ctypes.byref(h)))
+        self.handle = h
+        self.engine = engine
+
+    def sync(self):
+        _check(lib().cupoly_ctx_sync(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().cupoly_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _need(t, nres, ng, batch=None):
+    shape = (nres, ng, 2) if batch is None else (batch, nres, ng, 2)
+    if tuple(t.shape) != shape:
+        raise ValueError("cupoly_amd: tensor shape %s, expected %s" % (tuple(t.shape), shape))
+
+
+def execute_dgt(ctx, data, base, direction):
+    e = ctx.engine
+    _need(data, e.get_n_residues(base), e.N)
+    _check(lib().cupoly_dgt(ctx.handle, _ptr(data), base, direction))
+
+
+def _pw(fn, ctx, base, *ts):
+    e = ctx.engine
+    for t in ts:
+        _need(t, e.get_n_residues(base), e.N)
+    _check(fn(ctx.handle, *[_ptr(t) for t in ts], base))
+
+
+def execute_add_dgt(ctx, c, a, b, base=QBase):
+    _pw(lib().cupoly_add_dgt, ctx, base, c, a, b)
+
+
+def execute_sub_dgt(ctx, c, a, b, base=QBase):
+    _pw(lib().cupoly_sub_dgt, ctx, base, c, a, b)
+
+
+def execute_mul_dgt_gi(ctx, c, a, b, base=QBase):
+    _pw(lib().cupoly_mul_dgt_gi, ctx, base, c, a, b)
+
+
+def execute_muladd_dgt(ctx, d, a, b, c, base=QBase):
+    _pw(lib().cupoly_muladd_dgt, ctx, base, d, a, b, c)
+
+
+def execute_mul_dgt_u64(ctx, c, a, b, base=QBase):
+    e = ctx.engine
+    _need(c, e.get_n_residues(base), e.N)
+    _need(a, e.get_n_residues(base), e.N)
+    if tuple(b.shape) != (e.get_n_residues(base), e.N):
+        raise ValueError("cupoly_amd: mul_dgt_u64 operand shape")
+    _check(lib().cupoly_mul_dgt_u64(ctx.handle, _ptr(c), _ptr(a), _ptr(b), base))
+
+
+def execute_double_add_dgt(ctx, c1, a1, b1, c2, a2, b2, base=QBase):
+    _pw(lib().cupoly_double_add_dgt, ctx, base, c1, a1, b1, c2, a2, b2)
+
+
+def execute_dr2_dgt(ctx, ct21, ct22, ct23, ct01, ct02, ct11, ct12, base=QBase):
+    _pw(lib().cupoly_dr2_dgt, ctx, base, ct21, ct22, ct23, ct01, ct02, ct11, ct12)
+
+
+def execute_polynomial_negate(ctx, b, a, base=QBase):
+    _pw(lib().cupoly_negate, ctx, base, b, a)
+
+
+def execute_polynomial_op_by_int(ctx, b, a, x, op, base=QBase):
+    e = ctx.engine
+    _need(a, e.get_n_residues(base), e.N)
+    _need(b, e.get_n_residues(base), e.N)
+    _check(lib().cupoly_op_by_int(ctx.handle, _ptr(b), _ptr(a), x, op, base))
+
+
+def execute_polynomial_double_op_by_int(ctx, b1, a1, x1, b2, a2, x2, op, base=QBase):
+    _check(lib().cupoly_double_op_by_int(ctx.handle, _ptr(b1), _ptr(a1), x1, _ptr(b2),
+                                         _ptr(a2), x2, op, base))
+
+
+def execute_polynomial_basis_ext_Q_to_B(ctx, a):
+    """In place: a is [max(k, kl), N, 2]; Q residues in, B residues out."""
+    e = ctx.engine
+    if a.shape[0] < max(e.k, e.kl) or tuple(a.shape[1:]) != (e.N, 2):
+        raise ValueError("cupoly_amd: basis_ext_Q_to_B buffer shape")
+    _check(lib().cupoly_basis_ext_q_to_b(ctx.handle, _ptr(a)))
+
+
+def execute_polynomial_complex_scaling(ctx, b, a_q, a_b):
+    e = ctx.engine
+    _need(a_q, e.k, e.N)
+    _need(a_b, e.kl, e.N)
+    if tuple(b.shape) != (e.kl, e.n):
+        raise ValueError("cupoly_amd: complex_scaling output shape")
+    _check(lib().cupoly_complex_scaling(ctx.handle, _ptr(b), _ptr(a_q), _ptr(a_b)))
+
+
+def execute_polynomial_basis_ext_B_to_Q(ctx, b, a):
+    e = ctx.engine
+    _need(b, e.k, e.N)
+    if tuple(a.shape) != (e.kl, e.n):
+        raise ValueError("cupoly_amd: basis_ext_B_to_Q input shape")
+    _check(lib().cupoly_basis_ext_b_to_q(ctx.handle, _ptr(b), _ptr(a)))
+
+
+def execute_polynomial_simple_scaling(ctx, a):
+    e = ctx.engine
+    _need(a, e.k, e.N)
+    _check(lib().cupoly_simple_scaling(ctx.handle, _ptr(a)))
+
+
+def poly_basis_extension_Q_to_B(ctx, a_b, a_q):
+    e = ctx.engine
+    _need(a_q, e.k, e.N)
+    _need(a_b, e.kl, e.N)
+    _check(lib().cupoly_poly_basis_ext_q_to_b(ctx.handle, _ptr(a_b), _ptr(a_q)))
+
+
+def poly_complex_scaling_tDivQ(ctx, b_q, a_q, a_b):
+    e = ctx.engine
+    _need(a_q, e.k, e.N)
+    _need(b_q, e.k, e.N)
+    _need(a_b, e.kl, e.N)
+    _check(lib().cupoly_poly_complex_scaling(ctx.handle, _ptr(b_q), _ptr(a_q), _ptr(a_b)))
+
+
+class Sampler:
+    """Counter-based sampler (Philox4x32-10); seed and stream name the draw."""
+
+    @staticmethod
+    def sample(ctx, out, kind, seed, stream, base=QBase, coeff_domain=False):
+        e = ctx.engine
+        _need(out, e.get_n_residues(base), e.N)
+        fn = lib().cupoly_sample_coeff if coeff_domain else lib().cupoly_sample
+        _check(fn(ctx.handle, _ptr(out), kind, seed, stream, base))
+
+
+def polymul_batch(ctx, out, a, b):
+    """Fused coefficient-domain products out[i] = a[i] * b[i], [batch, k, N, 2]."""
+    e = ctx.engine
+    batch = a.shape[0]
+    for t in (out, a, b):
+        _need(t, e.k, e.N, batch)
+    _check(lib().cupoly_polymul_batch(ctx.handle, _ptr(out), _ptr(a), _ptr(b), batch))
+
+
+def polymul_plain(ctx, out, a, b):
+    """The same products one launch per op; a and b end in the DGT domain."""
+    e = ctx.engine
+    batch = a.shape[0]
+    for t in (out, a, b):
+        _need(t, e.k, e.N, batch)
+    _check(lib().cupoly_polymul_plain(ctx.handle, _ptr(out), _ptr(a), _ptr(b), batch))
+
+
+# ---- host <-> device helpers ------------------------------------------------
+def to_device(arr, device="cuda"):
+    """numpy uint64 array -> device tensor (int64 storage, same bits)."""
+    import numpy as np
+    import torch
+    a = np.ascontiguousarray(np.asarray(arr, dtype=np.uint64))
+    return torch.from_numpy(a.view(np.int64)).to(device)
+
+
+def to_host(t):
+    """device tensor (int64 storage) -> numpy uint64."""
+    import numpy as np
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def empty(shape, device="cuda"):
+    import torch
+    return torch.zeros(shape, dtype=torch.int64, device=device)

@@ -1,0 +1,743 @@
+// Discrete Galois Transform kernels for gfx950.
+//
+// What is computed (the reference's DGTEngine::execute_dgt,
+// src/cuda/dgt.cu:681-1094): for each residue, A_n = a_n + i a_{n+N} is
+// twisted by xi^n (xi^N = i) and transformed with omega = PROOT^((p-1)/N);
+// X_k = sum_n A_n xi^n omega^(nk) lands at position bitrev_N(k).  The
+// reference's hierarchical Na x Nb form places X_k at
+// bitrev_Na(k/Nb) + Na*bitrev_Nb(k%Nb), which is the same position, so one
+// radix-2 decimation-in-frequency network in natural order reproduces it.
+// The inverse is the decimation-in-time network with omega^-1 followed by the
+// untwist xi^-n / N.
+//
+// How (designed for CDNA): one workgroup owns a whole residue polynomial (up
+// to N = 8192 Gaussian slots = 128 KiB of LDS) or a 2^S-column / block slice
+// of a larger one.  Each thread keeps 8 slots in VGPRs and runs three radix-2
+// stages on them per pass ("radix-8 pass"); passes exchange data through LDS
+// with one barrier pair.  Global loads/stores are issued in the natural
+// layout (lane-contiguous 16-byte slots).  Twiddle and twist factors are
+// resident tables with Shoup quotients, so every constant product costs one
+// mulhi and two mullo on the integer pipes.  The last forward stage (and the
+// first inverse stage) has twiddle 1 and is done without a multiply.  The
+// fused product kernel keeps a's spectrum in registers while b is
+// transformed, multiplies slot-wise and runs the inverse without leaving the
+// workgroup: HBM traffic is the 24*k*n bytes of a, b and c.
+//
+// Sizes: N = 16..8192 in one kernel; N = 16384, 32768 in two kernels
+// (stages [0,S) on strided columns, then [S, log N) on contiguous blocks).
+#include "internal.hpp"
+
+#define EPT 8       // Gaussian slots per thread
+#define LOG_EPT 3
+
+// -------------------------------------------------------------------------
+// Index maps: local slot index (inside the workgroup's LDS array) -> global
+// slot index inside one residue polynomial.
+// -------------------------------------------------------------------------
+struct MapIdentity {  // whole polynomial, local == global
+  __device__ __forceinline__ int operator()(int l) const { return l; }
+};
+template <int LOGC, int LOGNS>  // 2^LOGC columns of a polynomial, column stride 2^LOGNS
+struct MapColumns {
+  int col0;
+  __device__ __forceinline__ int operator()(int l) const {
+    return col0 + (l & ((1 << LOGC) - 1)) + ((l >> LOGC) << LOGNS);
+  }
+};
+struct MapBlock {  // contiguous block starting at base
+  int base;
+  __device__ __forceinline__ int operator()(int l) const { return base + l; }
+};
+
+__device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, int e, uint64_t &w,
+                                      uint64_t &wp) {
+  const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(tw + 2 * e);
+  w = v.x;
+  wp = v.y;
+}
+
+// One forward pass over local stages [LS0, LS0+R) of a 2^LOGM local array.
+// Global stage = local stage + SOFF; LOGN = log2 of the global transform size.
+template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
+__device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+                                         uint64_t p, const Map &map) {
+  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
+#pragma unroll
+  for (int ju = 0; ju < NU; ju++) {
+    const int u = t + ju * T;
+    const int base = (u / ST) * (U * ST) + (u % ST);
+#pragma unroll
+    for (int h = 0; h < R; h++) {
+      const int half = U >> (h + 1);
+      const int s = LS0 + h + SOFF;
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        if (q & half) continue;
+        cp_gi &A = x[ju * U + q];
+        cp_gi &B = x[ju * U + q + half];
+        const uint64_t sr = A.re + B.re, si = A.imag + B.imag;
+        const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
+        A.re = cp_red2(sr, p);
+        A.imag = cp_red2(si, p);
+        if (s == LOGN - 1) {  // twiddle 1
+          B.re = cp_red2(dr, p);
+          B.imag = cp_red2(di, p);
+        } else {
+          const int g = map(base + q * ST);
+          const int e = (g & ((1 << (LOGN - 1 - s)) - 1)) << s;
+          uint64_t w, wp;
+          ld_tw(tw, e, w, wp);
+          B.re = cp_shoup(dr, w, wp, p);
+          B.imag = cp_shoup(di, w, wp, p);
+        }
+      }
+    }
+  }
+}
+
+// One inverse pass: same units, stages in reverse, DIT butterflies with omega^-1.
+template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
+__device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+                                         uint64_t p, const Map &map) {
+  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
+#pragma unroll
+  for (int ju = 0; ju < NU; ju++) {
+    const int u = t + ju * T;
+    const int base = (u / ST) * (U * ST) + (u % ST);
+#pragma unroll
+    for (int h = R - 1; h >= 0; h--) {
+      const int half = U >> (h + 1);
+      const int s = LS0 + h + SOFF;
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        if (q & half) continue;
+        cp_gi &A = x[ju * U + q];
+        cp_gi &B = x[ju * U + q + half];
+        uint64_t tr, ti;
+        if (s == LOGN - 1) {
+          tr = B.re;
+          ti = B.imag;
+        } else {
+          const int g = map(base + q * ST);
+          const int e = (g & ((1 << (LOGN - 1 - s)) - 1)) << s;
+          uint64_t w, wp;
+          ld_tw(tw, e, w, wp);
+          tr = cp_shoup(B.re, w, wp, p);
+          ti = cp_shoup(B.imag, w, wp, p);
+        }
+        B.re = cp_red2(A.re + p - tr, p);
+        B.imag = cp_red2(A.imag + p - ti, p);
+        A.re = cp_red2(A.re + tr, p);
+        A.imag = cp_red2(A.imag + ti, p);
+      }
+    }
+  }
+}
+
+// Local index of register slot i in the layout of the pass starting at LS0 with R stages.
+template <int LOGM, int LS0, int R>
+__device__ __forceinline__ int pass_index(int t, int i) {
+  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R);
+  const int ju = i / U, q = i % U;
+  const int u = t + ju * T;
+  return (u / ST) * (U * ST) + (u % ST) + q * ST;
+}
+
+// Same for a full radix-8 pass (R = 3, one unit per thread) starting at a
+// runtime local stage ls0: unit stride ST = M >> (ls0 + 3).
+template <int LOGM>
+__device__ __forceinline__ int pass_index_rt(int t, int i, int ls0) {
+  const int lst = LOGM - ls0 - LOG_EPT;  // log2(ST)
+  return ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1)) + (i << lst);
+}
+
+template <int LOGM, int LS0, int R>
+__device__ __forceinline__ void lds_store(cp_gi *lds, const cp_gi (&x)[EPT], int t) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) lds[pass_index<LOGM, LS0, R>(t, i)] = x[i];
+}
+template <int LOGM, int LS0, int R>
+__device__ __forceinline__ void lds_load(const cp_gi *lds, cp_gi (&x)[EPT], int t) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index<LOGM, LS0, R>(t, i)];
+}
+template <int LOGM>
+__device__ __forceinline__ void lds_store_rt(cp_gi *lds, const cp_gi (&x)[EPT], int t, int ls0) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) lds[pass_index_rt<LOGM>(t, i, ls0)] = x[i];
+}
+template <int LOGM>
+__device__ __forceinline__ void lds_load_rt(const cp_gi *lds, cp_gi (&x)[EPT], int t, int ls0) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index_rt<LOGM>(t, i, ls0)];
+}
+
+// Full radix-8 pass at a runtime local stage ls0 (never the last global
+// stage, so every twiddle is a real multiply).
+template <int LOGN, int LOGM, int SOFF, class Map>
+__device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+                                            uint64_t p, const Map &map, int ls0) {
+  const int lst = LOGM - ls0 - LOG_EPT;
+  const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+#pragma unroll
+  for (int h = 0; h < LOG_EPT; h++) {
+    const int half = EPT >> (h + 1);
+    const int s = ls0 + h + SOFF;
+    const int mmask = (1 << (LOGN - 1 - s)) - 1;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      if (q & half) continue;
+      cp_gi &A = x[q];
+      cp_gi &B = x[q + half];
+      const uint64_t sr = A.re + B.re, si = A.imag + B.imag;
+      const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
+      A.re = cp_red2(sr, p);
+      A.imag = cp_red2(si, p);
+      const int e = (map(base + (q << lst)) & mmask) << s;
+      uint64_t w, wp;
+      ld_tw(tw, e, w, wp);
+      B.re = cp_shoup(dr, w, wp, p);
+      B.imag = cp_shoup(di, w, wp, p);
+    }
+  }
+}
+
+template <int LOGN, int LOGM, int SOFF, class Map>
+__device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+                                            uint64_t p, const Map &map, int ls0) {
+  const int lst = LOGM - ls0 - LOG_EPT;
+  const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+#pragma unroll
+  for (int h = LOG_EPT - 1; h >= 0; h--) {
+    const int half = EPT >> (h + 1);
+    const int s = ls0 + h + SOFF;
+    const int mmask = (1 << (LOGN - 1 - s)) - 1;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      if (q & half) continue;
+      cp_gi &A = x[q];
+      cp_gi &B = x[q + half];
+      const int e = (map(base + (q << lst)) & mmask) << s;
+      uint64_t w, wp;
+      ld_tw(tw, e, w, wp);
+      const uint64_t tr = cp_shoup(B.re, w, wp, p);
+      const uint64_t ti = cp_shoup(B.imag, w, wp, p);
+      B.re = cp_red2(A.re + p - tr, p);
+      B.imag = cp_red2(A.imag + p - ti, p);
+      A.re = cp_red2(A.re + tr, p);
+      A.imag = cp_red2(A.imag + ti, p);
+    }
+  }
+}
+
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// Pass plan over local stages [LSB, LSE): NP passes; all but the last are
+// full radix-8 passes run in a loop that is not unrolled (so the compiler
+// cannot hoist twiddle loads across passes); the last pass has RL stages.
+template <int LSB, int LSE>
+struct PassPlan {
+  static constexpr int NS = LSE - LSB;
+  static constexpr int NP = (NS + LOG_EPT - 1) / LOG_EPT;
+  static constexpr int LSL = LSB + LOG_EPT * (NP - 1);  // first stage of the last pass
+  static constexpr int RL = LSE - LSL;                  // stages in the last pass
+};
+
+// Forward passes over local stages [LSB, LSE), starting from registers in the
+// layout of the first pass (natural layout when LSB == 0), ending in the
+// layout of the last pass.
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
+__device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
+                                           const uint64_t *__restrict__ tw, uint64_t p,
+                                           const Map &map) {
+  using PP = PassPlan<LSB, LSE>;
+#pragma nounroll
+  for (int ps = 0; ps < PP::NP - 1; ps++) {
+    const int ls0 = LSB + LOG_EPT * ps;
+    dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
+    __syncthreads();
+    lds_store_rt<LOGM>(lds, x, t, ls0);
+    __syncthreads();
+    if (ps + 1 < PP::NP - 1)
+      lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
+    else
+      lds_load<LOGM, PP::LSL, PP::RL>(lds, x, t);
+  }
+  dif_pass<LOGN, LOGM, PP::LSL, PP::RL, SOFF>(x, t, tw, p, map);
+}
+
+// Inverse passes over local stages [LSB, LSE) in reverse order; registers
+// start in the layout of the last pass and end in the layout of the first.
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
+__device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
+                                           const uint64_t *__restrict__ tw, uint64_t p,
+                                           const Map &map) {
+  using PP = PassPlan<LSB, LSE>;
+  dit_pass<LOGN, LOGM, PP::LSL, PP::RL, SOFF>(x, t, tw, p, map);
+#pragma nounroll
+  for (int ps = PP::NP - 2; ps >= 0; ps--) {
+    const int ls0 = LSB + LOG_EPT * ps;
+    __syncthreads();
+    if (ps + 1 == PP::NP - 1)
+      lds_store<LOGM, PP::LSL, PP::RL>(lds, x, t);
+    else
+      lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
+    __syncthreads();
+    lds_load_rt<LOGM>(lds, x, t, ls0);
+    dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
+  }
+}
+
+// Layout of the last pass over [LSB, LSE).
+template <int LOGM, int LSB, int LSE>
+struct LastPass {
+  static constexpr int LS0 = PassPlan<LSB, LSE>::LSL;
+  static constexpr int RL = PassPlan<LSB, LSE>::RL;
+};
+
+// natural layout index of register i: t + i*T (== first-pass layout)
+template <int LOGM>
+__device__ __forceinline__ int nat_index(int t, int i) {
+  return t + i * ((1 << LOGM) / EPT);
+}
+
+__device__ __forceinline__ void twist(cp_gi &v, const uint64_t *__restrict__ tab, int g, uint64_t p) {
+  const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(tab + 6 * (size_t)g);
+  const ulonglong2 a = k2[0], b = k2[1], c = k2[2];
+  const uint64_t k[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+  v = cp_gimul_const(v, k, p);
+}
+
+// -------------------------------------------------------------------------
+// Single-kernel transform: N = 2^LOGN <= 8192.  ROWS residue polynomials per
+// workgroup; unit u = blockIdx.x*ROWS + row is (poly u / nres, residue u % nres).
+// -------------------------------------------------------------------------
+template <int LOGN, int ROWS, bool FWD>
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+    k_dgt_single(cp_gi *__restrict__ data, int nunits, int nres, int roff,
+                 const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all,
+                 const uint64_t *__restrict__ twist_all) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  extern __shared__ cp_gi lds_all[];
+  const int row = threadIdx.x / T, t = threadIdx.x % T;
+  cp_gi *lds = lds_all + row * N;
+  int unit = blockIdx.x * ROWS + row;
+  const bool active = unit < nunits;
+  if (!active) unit = nunits - 1;
+  const int rid = roff + unit % nres;
+  const uint64_t p = primes[rid].p;
+  const uint64_t *tw = tw_all + (size_t)rid * N;          // N/2 pairs
+  const uint64_t *tt = twist_all + (size_t)rid * N * 6;
+  cp_gi *poly = data + (size_t)unit * N;
+  cp_gi x[EPT];
+  MapIdentity id;
+  if constexpr (FWD) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = nat_index<LOGN>(t, i);
+      x[i] = poly[g];
+      twist(x[i], tt, g, p);
+    }
+    dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
+    using LP = LastPass<LOGN, 0, LOGN>;
+    __syncthreads();
+    lds_store<LOGN, LP::LS0, LP::RL>(lds, x, t);
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < EPT; i++) poly[nat_index<LOGN>(t, i)] = lds[nat_index<LOGN>(t, i)];
+    }
+  } else {
+    using LP = LastPass<LOGN, 0, LOGN>;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) lds[nat_index<LOGN>(t, i)] = poly[nat_index<LOGN>(t, i)];
+    __syncthreads();
+    lds_load<LOGN, LP::LS0, LP::RL>(lds, x, t);
+    dit_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < EPT; i++) {
+        const int g = nat_index<LOGN>(t, i);
+        twist(x[i], tt, g, p);
+        poly[g] = x[i];
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
+// Two-kernel transform for N = 2^LOGN > 8192: S = LOGN - 7 column stages
+// handled as 2^LOGC consecutive columns per workgroup; the remaining 7 stages
+// on contiguous blocks of 128 slots, 2^(13-7) blocks per workgroup.
+// -------------------------------------------------------------------------
+template <int LOGN>
+struct Split {
+  static constexpr int LOGB = 7;                 // block size 128
+  static constexpr int S = LOGN - LOGB;          // column stages; column length 2^S
+  static constexpr int LOGNS = LOGN - S;         // column stride = 2^LOGB
+  static constexpr int LOGC = 13 - S;            // columns per workgroup
+  static constexpr int LOGM = 13;                // local array 8192 slots
+  static constexpr int WG_PER_POLY = (1 << LOGNS) >> LOGC;
+  static constexpr int BLOCKS_PER_WG = (1 << LOGM) >> LOGB;
+};
+
+// Column stages [0, S): forward (twist first) or inverse (untwist last).
+template <int LOGN, bool FWD>
+__global__ void __launch_bounds__(1024)
+    k_dgt_cols(cp_gi *__restrict__ data, int nunits, int nres, int roff,
+               const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all,
+               const uint64_t *__restrict__ twist_all) {
+  using SP = Split<LOGN>;
+  constexpr int N = 1 << LOGN, LOGM = SP::LOGM;
+  extern __shared__ cp_gi lds[];
+  const int t = threadIdx.x;
+  const int unit = blockIdx.x / SP::WG_PER_POLY;
+  const int col0 = (blockIdx.x % SP::WG_PER_POLY) << SP::LOGC;
+  const int rid = roff + unit % nres;
+  const uint64_t p = primes[rid].p;
+  const uint64_t *tw = tw_all + (size_t)rid * N;
+  const uint64_t *tt = twist_all + (size_t)rid * N * 6;
+  cp_gi *poly = data + (size_t)unit * N;
+  MapColumns<SP::LOGC, SP::LOGNS> map{col0};
+  cp_gi x[EPT];
+  // local stages [0, S) of the 2^13 local array are global stages [0, S)
+  if constexpr (FWD) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = map(nat_index<LOGM>(t, i));
+      x[i] = poly[g];
+      twist(x[i], tt, g, p);
+    }
+    dif_passes<LOGN, LOGM, 0, SP::S, 0>(x, lds, t, tw, p, map);
+    using LP = LastPass<LOGM, 0, SP::S>;
+    __syncthreads();
+    lds_store<LOGM, LP::LS0, LP::RL>(lds, x, t);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGM>(t, i))] = lds[nat_index<LOGM>(t, i)];
+  } else {
+    using LP = LastPass<LOGM, 0, SP::S>;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) lds[nat_index<LOGM>(t, i)] = poly[map(nat_index<LOGM>(t, i))];
+    __syncthreads();
+    lds_load<LOGM, LP::LS0, LP::RL>(lds, x, t);
+    dit_passes<LOGN, LOGM, 0, SP::S, 0>(x, lds, t, tw, p, map);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = map(nat_index<LOGM>(t, i));
+      twist(x[i], tt, g, p);
+      poly[g] = x[i];
+    }
+  }
+  (void)nunits;
+}
+
+// Block stages [S, LOGN): each workgroup row is one 128-slot block.
+template <int LOGN, bool FWD>
+__global__ void __launch_bounds__(1024)
+    k_dgt_blocks(cp_gi *__restrict__ data, int nunits, int nres, int roff,
+                 const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all) {
+  using SP = Split<LOGN>;
+  constexpr int N = 1 << LOGN, LOGB = SP::LOGB, B = 1 << LOGB, TB = B / EPT;
+  extern __shared__ cp_gi lds_all[];
+  const int row = threadIdx.x / TB, t = threadIdx.x % TB;
+  const int gblk = blockIdx.x * SP::BLOCKS_PER_WG + row;  // global block over all units
+  const int unit = gblk >> (LOGN - LOGB);
+  const int blk = gblk & ((1 << (LOGN - LOGB)) - 1);
+  const int rid = roff + unit % nres;
+  const uint64_t p = primes[rid].p;
+  const uint64_t *tw = tw_all + (size_t)rid * N;
+  cp_gi *lds = lds_all + row * B;
+  cp_gi *poly = data + (size_t)unit * N;
+  MapBlock map{blk << LOGB};
+  cp_gi x[EPT];
+  using LP = LastPass<LOGB, 0, LOGB>;
+  if constexpr (FWD) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) x[i] = poly[map(nat_index<LOGB>(t, i))];
+    dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
+    __syncthreads();
+    lds_store<LOGB, LP::LS0, LP::RL>(lds, x, t);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGB>(t, i))] = lds[nat_index<LOGB>(t, i)];
+  } else {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) lds[nat_index<LOGB>(t, i)] = poly[map(nat_index<LOGB>(t, i))];
+    __syncthreads();
+    lds_load<LOGB, LP::LS0, LP::RL>(lds, x, t);
+    dit_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGB>(t, i))] = x[i];
+  }
+  (void)nunits;
+}
+
+// -------------------------------------------------------------------------
+// Fused coefficient-domain product c = a*b, N <= 8192: forward(a) and
+// forward(b) in one workgroup, slot-wise Gaussian product, inverse.
+// -------------------------------------------------------------------------
+template <int LOGN, int ROWS>
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+    k_polymul_fused(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
+                    const cp_gi *__restrict__ b, int nunits, int nres,
+                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                    const uint64_t *__restrict__ twi_all, const uint64_t *__restrict__ tsf_all,
+                    const uint64_t *__restrict__ tsi_all) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  extern __shared__ cp_gi lds_all[];
+  const int row = threadIdx.x / T, t = threadIdx.x % T;
+  cp_gi *lds = lds_all + row * N;
+  int unit = blockIdx.x * ROWS + row;
+  const bool active = unit < nunits;
+  if (!active) unit = nunits - 1;
+  const int rid = unit % nres;
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
+  const uint64_t *tsf = tsf_all + (size_t)rid * N * 6, *tsi = tsi_all + (size_t)rid * N * 6;
+  const size_t off = (size_t)unit * N;
+  MapIdentity id;
+  cp_gi xa[EPT], xb[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = nat_index<LOGN>(t, i);
+    xa[i] = a[off + g];
+    xb[i] = b[off + g];
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; i++) twist(xa[i], tsf, nat_index<LOGN>(t, i), p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) twist(xb[i], tsf, nat_index<LOGN>(t, i), p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) xa[i] = cp_gimul(xa[i], xb[i], pr);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twi, p, id);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = nat_index<LOGN>(t, i);
+      twist(xa[i], tsi, g, p);
+      out[off + g] = xa[i];
+    }
+  }
+}
+
+// Fused block stages for N > 8192: forward blocks of a and b, product, inverse blocks.
+template <int LOGN>
+__global__ void __launch_bounds__(1024)
+    k_polymul_blocks(cp_gi *__restrict__ a, const cp_gi *__restrict__ b, int nres,
+                     const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                     const uint64_t *__restrict__ twi_all) {
+  using SP = Split<LOGN>;
+  constexpr int N = 1 << LOGN, LOGB = SP::LOGB, B = 1 << LOGB, TB = B / EPT;
+  extern __shared__ cp_gi lds_all[];
+  const int row = threadIdx.x / TB, t = threadIdx.x % TB;
+  const int gblk = blockIdx.x * SP::BLOCKS_PER_WG + row;
+  const int unit = gblk >> (LOGN - LOGB);
+  const int blk = gblk & ((1 << (LOGN - LOGB)) - 1);
+  const int rid = unit % nres;
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
+  cp_gi *lds = lds_all + row * B;
+  cp_gi *pa = a + (size_t)unit * N;
+  const cp_gi *pb = b + (size_t)unit * N;
+  MapBlock map{blk << LOGB};
+  cp_gi xa[EPT], xb[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    xa[i] = pa[map(nat_index<LOGB>(t, i))];
+    xb[i] = pb[map(nat_index<LOGB>(t, i))];
+  }
+  dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(xa, lds, t, twf, p, map);
+  dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(xb, lds, t, twf, p, map);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) xa[i] = cp_gimul(xa[i], xb[i], pr);
+  dit_passes<LOGN, LOGB, 0, LOGB, SP::S>(xa, lds, t, twi, p, map);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) pa[map(nat_index<LOGB>(t, i))] = xa[i];
+}
+
+// -------------------------------------------------------------------------
+// Host side
+// -------------------------------------------------------------------------
+template <int LOGN>
+constexpr int rows_for() {
+  return ((1 << LOGN) / EPT) >= 256 ? 1 : 256 / ((1 << LOGN) / EPT);
+}
+
+template <class K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    CP_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+  return CUPOLY_OK;
+}
+
+template <int LOGN>
+static int setup_size() {
+  int rc;
+  if constexpr (LOGN <= 13) {
+    constexpr int R = rows_for<LOGN>();
+    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    if ((rc = set_lds(k_dgt_single<LOGN, R, true>, lds))) return rc;
+    if ((rc = set_lds(k_dgt_single<LOGN, R, false>, lds))) return rc;
+    if ((rc = set_lds(k_polymul_fused<LOGN, R>, lds))) return rc;
+  } else {
+    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    if ((rc = set_lds(k_dgt_cols<LOGN, true>, lds))) return rc;
+    if ((rc = set_lds(k_dgt_cols<LOGN, false>, lds))) return rc;
+    if ((rc = set_lds(k_dgt_blocks<LOGN, true>, lds))) return rc;
+    if ((rc = set_lds(k_dgt_blocks<LOGN, false>, lds))) return rc;
+    if ((rc = set_lds(k_polymul_blocks<LOGN>, lds))) return rc;
+  }
+  return CUPOLY_OK;
+}
+
+int cp_dgt_setup(cupoly_engine *e) {
+  switch (e->logng) {
+    case 4: return setup_size<4>();
+    case 5: return setup_size<5>();
+    case 6: return setup_size<6>();
+    case 7: return setup_size<7>();
+    case 8: return setup_size<8>();
+    case 9: return setup_size<9>();
+    case 10: return setup_size<10>();
+    case 11: return setup_size<11>();
+    case 12: return setup_size<12>();
+    case 13: return setup_size<13>();
+    case 14: return setup_size<14>();
+    case 15: return setup_size<15>();
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "DGT is not working for such N");
+  }
+}
+
+template <int LOGN>
+static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int dir) {
+  cupoly_engine *e = c->e;
+  const int nunits = nres * npoly;
+  if (nunits <= 0) return CUPOLY_OK;
+  if constexpr (LOGN <= 13) {
+    constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
+    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    const int grid = (nunits + R - 1) / R;
+    if (dir == CUPOLY_FORWARD)
+      hipLaunchKernelGGL((k_dgt_single<LOGN, R, true>), dim3(grid), dim3(T * R), lds, c->stream,
+                         data, nunits, nres, roff, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+    else
+      hipLaunchKernelGGL((k_dgt_single<LOGN, R, false>), dim3(grid), dim3(T * R), lds, c->stream,
+                         data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+  } else {
+    using SP = Split<LOGN>;
+    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    const int gcols = nunits * SP::WG_PER_POLY;
+    const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
+    if (dir == CUPOLY_FORWARD) {
+      hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, data,
+                         nunits, nres, roff, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+      hipLaunchKernelGGL((k_dgt_blocks<LOGN, true>), dim3(gblks), dim3(1024), lds, c->stream,
+                         data, nunits, nres, roff, e->d_primes, e->d_tw_fwd);
+    } else {
+      hipLaunchKernelGGL((k_dgt_blocks<LOGN, false>), dim3(gblks), dim3(1024), lds, c->stream,
+                         data, nunits, nres, roff, e->d_primes, e->d_tw_inv);
+      hipLaunchKernelGGL((k_dgt_cols<LOGN, false>), dim3(gcols), dim3(1024), lds, c->stream,
+                         data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+    }
+  }
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
+int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int dir) {
+  switch (c->e->logng) {
+    case 4: return dgt_launch_t<4>(c, data, roff, nres, npoly, dir);
+    case 5: return dgt_launch_t<5>(c, data, roff, nres, npoly, dir);
+    case 6: return dgt_launch_t<6>(c, data, roff, nres, npoly, dir);
+    case 7: return dgt_launch_t<7>(c, data, roff, nres, npoly, dir);
+    case 8: return dgt_launch_t<8>(c, data, roff, nres, npoly, dir);
+    case 9: return dgt_launch_t<9>(c, data, roff, nres, npoly, dir);
+    case 10: return dgt_launch_t<10>(c, data, roff, nres, npoly, dir);
+    case 11: return dgt_launch_t<11>(c, data, roff, nres, npoly, dir);
+    case 12: return dgt_launch_t<12>(c, data, roff, nres, npoly, dir);
+    case 13: return dgt_launch_t<13>(c, data, roff, nres, npoly, dir);
+    case 14: return dgt_launch_t<14>(c, data, roff, nres, npoly, dir);
+    case 15: return dgt_launch_t<15>(c, data, roff, nres, npoly, dir);
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "DGT is not working for such N");
+  }
+}
+
+template <int LOGN>
+static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch) {
+  cupoly_engine *e = c->e;
+  const int nres = e->k, nunits = nres * batch;
+  if (nunits <= 0) return CUPOLY_OK;
+  if constexpr (LOGN <= 13) {
+    constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
+    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
+                       c->stream, out, a, b, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
+                       e->d_twist_fwd, e->d_twist_inv);
+    CP_LAUNCH_CHECK();
+  } else {
+    // out <- a, scratch <- b (caller inputs are not modified); transforms in place
+    using SP = Split<LOGN>;
+    const size_t bytes = (size_t)nunits * (1 << LOGN) * sizeof(cp_gi);
+    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    if (out != a) CP_HIP(hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream));
+    cp_gi *tb = nullptr;
+    CP_HIP(hipMallocAsync((void **)&tb, bytes, c->stream));
+    CP_HIP(hipMemcpyAsync(tb, b, bytes, hipMemcpyDeviceToDevice, c->stream));
+    const int gcols = nunits * SP::WG_PER_POLY;
+    const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
+    hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, out,
+                       nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+    hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, tb,
+                       nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+    hipLaunchKernelGGL((k_polymul_blocks<LOGN>), dim3(gblks), dim3(1024), lds, c->stream, out,
+                       tb, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv);
+    hipLaunchKernelGGL((k_dgt_cols<LOGN, false>), dim3(gcols), dim3(1024), lds, c->stream, out,
+                       nunits, nres, 0, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+    CP_LAUNCH_CHECK();
+    CP_HIP(hipFreeAsync(tb, c->stream));
+  }
+  return CUPOLY_OK;
+}
+
+int cp_polymul_fused_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch) {
+  switch (c->e->logng) {
+    case 4: return polymul_t<4>(c, out, a, b, batch);
+    case 5: return polymul_t<5>(c, out, a, b, batch);
+    case 6: return polymul_t<6>(c, out, a, b, batch);
+    case 7: return polymul_t<7>(c, out, a, b, batch);
+    case 8: return polymul_t<8>(c, out, a, b, batch);
+    case 9: return polymul_t<9>(c, out, a, b, batch);
+    case 10: return polymul_t<10>(c, out, a, b, batch);
+    case 11: return polymul_t<11>(c, out, a, b, batch);
+    case 12: return polymul_t<12>(c, out, a, b, batch);
+    case 13: return polymul_t<13>(c, out, a, b, batch);
+    case 14: return polymul_t<14>(c, out, a, b, batch);
+    case 15: return polymul_t<15>(c, out, a, b, batch);
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "DGT is not working for such N");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int cupoly_dgt(cupoly_ctx *c, cupoly_gi *data, int base, int direction) {
+  if (!c || !data) return cp_fail(CUPOLY_ERR_ARG, "dgt: null argument");
+  if (base == CUPOLY_TBASE) return CUPOLY_OK;  // dgt.cu:1084-1085
+  int off, cnt;
+  if (!cp_base_range(c->e, base, &off, &cnt)) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "Unknown base");
+  if (direction != CUPOLY_FORWARD && direction != CUPOLY_INVERSE)
+    return cp_fail(CUPOLY_ERR_ARG, "dgt: bad direction");
+  return cp_dgt_launch(c, (cp_gi *)data, off, cnt, 1, direction);
+}
+
+extern "C" int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
+                                    const cupoly_gi *b, int batch) {
+  if (!c || !out || !a || !b || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "polymul_batch: bad argument");
+  return cp_polymul_fused_launch(c, (cp_gi *)out, (const cp_gi *)a, (const cp_gi *)b, batch);
+}

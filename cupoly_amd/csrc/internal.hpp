@@ -1,0 +1,74 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cupoly_amd.h"
+#include "modarith.hpp"
+
+#define CP_MAX_RES 30   // primes in the BFV bucket (settings.h:108-110)
+#define CP_MAX_BASE 14  // residues per base: MAX_COPRIMES_IN_A_BASE - 1 (cudaengine.cu:1270-1276)
+
+// HPS constants, one device copy per engine.  Shoup pairs are {w, w'}.
+struct cp_hps {
+  int k, kl;
+  uint64_t t;
+  cp_prime q[CP_MAX_BASE], b[CP_MAX_BASE];
+  uint64_t invQqi[CP_MAX_BASE][2];               // (Q/q_j)^-1 mod q_j
+  uint64_t Qqibi[CP_MAX_BASE][CP_MAX_BASE][2];   // [i][j] (Q/q_j) mod b_i
+  uint64_t Qbi[CP_MAX_BASE][2];                  // Q mod b_i
+  double rcp_q[CP_MAX_BASE];                     // RD(1 / RN(q_j))
+  uint64_t invBbi[CP_MAX_BASE][2];               // (B/b_i)^-1 mod b_i
+  uint64_t Bbiqi[CP_MAX_BASE][CP_MAX_BASE][2];   // [j][i] (B/b_i) mod q_j
+  uint64_t Bqi[CP_MAX_BASE][2];                  // B mod q_j
+  uint64_t wint[CP_MAX_BASE][CP_MAX_BASE][2];    // [i][j] omega_int mod b_i
+  uint64_t wfrac[CP_MAX_BASE][2];                // [j] {lo, hi} of omega_frac
+  uint64_t lam[CP_MAX_BASE][2];                  // lambda_i
+  uint64_t twint[CP_MAX_BASE];                   // simple scaling, integer part
+  uint64_t twfrac[CP_MAX_BASE][2];               // simple scaling, fraction * 2^128
+};
+
+struct cupoly_engine {
+  int k = 0, kl = 0, n = 0, ng = 0, logng = 0, nres = 0;
+  uint64_t t = 0;
+  int device = 0;
+  std::vector<uint64_t> primes;      // Q then B
+  cp_prime *d_primes = nullptr;      // [nres]
+  // DGT tables, per residue r (all of Q then B):
+  uint64_t *d_tw_fwd = nullptr;      // [nres][ng/2][2]   omega^e, Shoup
+  uint64_t *d_tw_inv = nullptr;      // [nres][ng/2][2]   omega^-e, Shoup
+  uint64_t *d_twist_fwd = nullptr;   // [nres][ng][6]     xi^n as three Shoup pairs
+  uint64_t *d_twist_inv = nullptr;   // [nres][ng][6]     xi^-n / ng
+  cp_hps h_hps;                      // host copy
+  cp_hps *d_hps = nullptr;
+};
+
+struct cupoly_ctx {
+  cupoly_engine *e = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint64_t *d_aux = nullptr;   // uint64 [kl][n]  (ctx->d_coefs_B in the reference)
+  cp_gi *d_tmp = nullptr;      // [nres][ng]      (ctx->d_tmp_data / d_aux_coefs)
+  cp_gi *d_tmp2 = nullptr;     // [nres][ng]
+};
+
+// error plumbing
+int cp_fail(int code, const std::string &msg);
+int cp_check_hip(hipError_t err, const char *what);
+#define CP_HIP(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return cp_check_hip(_e, #call); \
+  } while (0)
+#define CP_LAUNCH_CHECK() CP_HIP(hipGetLastError())
+
+// base -> (first residue index, residue count); returns false on bad base
+bool cp_base_range(const cupoly_engine *e, int base, int *off, int *cnt);
+
+// kernel launchers implemented in the .hip files
+int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int direction);
+int cp_polymul_fused_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch);
+int cp_dgt_setup(cupoly_engine *e);

@@ -1,0 +1,127 @@
+// Modular arithmetic on 63-bit RNS primes, shared by host and device code.
+//
+// The reference reduces every 64x64 product with a Barrett step
+// (src/cuda/dgt.cu:394-466) and multiplies Gaussian integers with Karatsuba
+// (dgt.cu:547-570).  Its results are exact residues in [0, p), so any exact
+// reduction reproduces them bit for bit.  Here:
+//   * products by a constant known in advance (twiddles, twists, RNS
+//     constants) use Shoup's precomputed quotient w' = floor(w * 2^64 / p):
+//     one high and two low 64-bit products, valid for any 64-bit input;
+//   * products of two variables use Moller-Granlund division by the
+//     normalised divisor 2p with a precomputed inverse ("Improved division by
+//     invariant integers", IEEE TC 2011, Alg. 4), then one correction.
+// Both need p in (2^62, 2^63), which every prime of the 63-bit bucket meets
+// (the engine checks it).  On gfx950 the 128-bit products lower to
+// v_mad_u64_u32 / v_mul_hi_u32 chains on the integer pipes.
+#pragma once
+#include <stdint.h>
+
+#ifndef CP_HD
+#define CP_HD __host__ __device__ __forceinline__
+#endif
+
+typedef unsigned __int128 cp_u128;
+
+struct alignas(16) cp_gi {  // GaussianInteger {re, imag}, 16 bytes (include/cuPoly/settings.h:70-80)
+  uint64_t re, imag;
+};
+
+// Per-prime constants for exact reduction.
+struct cp_prime {
+  uint64_t p;     // the prime, 2^62 < p < 2^63
+  uint64_t d;     // 2p (normalised divisor, top bit set)
+  uint64_t dinv;  // floor((2^128 - 1) / d) - 2^64
+};
+
+CP_HD uint64_t cp_mulhi(uint64_t a, uint64_t b) {
+  return (uint64_t)(((cp_u128)a * b) >> 64);
+}
+
+CP_HD uint64_t cp_add(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t s = a + b;  // a, b < p < 2^63: no wrap
+  return s >= p ? s - p : s;
+}
+
+CP_HD uint64_t cp_sub(uint64_t a, uint64_t b, uint64_t p) {
+  return a >= b ? a - b : a + (p - b);
+}
+
+// Reduce from [0, 2p) to [0, p).
+CP_HD uint64_t cp_red2(uint64_t a, uint64_t p) { return a >= p ? a - p : a; }
+
+// Shoup: a * w mod p in [0, 2p) for any a < 2^64, w < p, wp = floor(w 2^64 / p).
+CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
+  uint64_t q = cp_mulhi(a, wp);
+  return a * w - q * p;
+}
+
+CP_HD uint64_t cp_shoup(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
+  return cp_red2(cp_shoup_lazy(a, w, wp, p), p);
+}
+
+// (u1:u0) mod pr.d for u1 < pr.d, via the precomputed inverse.
+CP_HD uint64_t cp_rem2(uint64_t u1, uint64_t u0, const cp_prime& pr) {
+  cp_u128 q = (cp_u128)pr.dinv * u1;
+  q += ((cp_u128)(u1 + 1) << 64) | u0;
+  uint64_t q1 = (uint64_t)(q >> 64), q0 = (uint64_t)q;
+  uint64_t r = u0 - q1 * pr.d;
+  if (r > q0) r += pr.d;
+  if (r >= pr.d) r -= pr.d;
+  return r;
+}
+
+// a * b mod p for a, b < 2^63 (so a*b < 2^126 and the high word < 2p).
+CP_HD uint64_t cp_mulmod(uint64_t a, uint64_t b, const cp_prime& pr) {
+  cp_u128 x = (cp_u128)a * b;
+  uint64_t r = cp_rem2((uint64_t)(x >> 64), (uint64_t)x, pr);
+  return r >= pr.p ? r - pr.p : r;
+}
+
+// Gaussian integer product (a.re + i a.im)(b.re + i b.im) mod p, Karatsuba
+// with three exact products as the reference's GIMul.
+CP_HD cp_gi cp_gimul(cp_gi a, cp_gi b, const cp_prime& pr) {
+  uint64_t p = pr.p;
+  uint64_t s1 = cp_mulmod(a.re, b.re, pr);
+  uint64_t s2 = cp_mulmod(a.imag, b.imag, pr);
+  uint64_t s3 = cp_mulmod(cp_add(a.re, a.imag, p), cp_add(b.re, b.imag, p), pr);
+  cp_gi c;
+  c.re = cp_sub(s1, s2, p);
+  c.imag = cp_sub(s3, cp_add(s1, s2, p), p);
+  return c;
+}
+
+// Gaussian product by a constant (c + i d) given as three Shoup pairs:
+// k[0..1] = (c, c'), k[2..3] = (d, d'), k[4..5] = (c+d mod p, (c+d)').
+CP_HD cp_gi cp_gimul_const(cp_gi a, const uint64_t* k, uint64_t p) {
+  uint64_t s1 = cp_shoup(a.re, k[0], k[1], p);
+  uint64_t s2 = cp_shoup(a.imag, k[2], k[3], p);
+  uint64_t s3 = cp_shoup(a.re + a.imag, k[4], k[5], p);  // a.re + a.imag < 2^64
+  cp_gi c;
+  c.re = cp_sub(s1, s2, p);
+  c.imag = cp_sub(s3, cp_add(s1, s2, p), p);
+  return c;
+}
+
+// Host-side helpers for table construction.
+static inline uint64_t cp_shoup_factor(uint64_t w, uint64_t p) {
+  return (uint64_t)(((cp_u128)w << 64) / p);
+}
+
+static inline cp_prime cp_make_prime(uint64_t p) {
+  cp_prime pr;
+  pr.p = p;
+  pr.d = p << 1;
+  cp_u128 all = ~(cp_u128)0;
+  pr.dinv = (uint64_t)(all / pr.d);  // floor((2^128-1)/d) - 2^64 (wraps the 2^64)
+  return pr;
+}
+
+static inline uint64_t cp_powmod_host(uint64_t a, uint64_t e, uint64_t p) {
+  uint64_t r = 1, s = a % p;
+  while (e) {
+    if (e & 1) r = (uint64_t)(((cp_u128)r * s) % p);
+    s = (uint64_t)(((cp_u128)s * s) % p);
+    e >>= 1;
+  }
+  return r;
+}

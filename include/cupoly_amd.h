@@ -1,0 +1,152 @@
+/*
+ * cupoly_amd.h - C ABI of the MI355X (gfx950) polynomial hot path.
+ *
+ * Every entry point replaces one interface of the reference library
+ * (pdroalves/cupoly); the comment on each cites the reference declaration it
+ * stands in for.  Plain pointers and sizes only: device buffers are HIP device
+ * pointers, streams are hipStream_t passed as void*.  All functions return
+ * CUPOLY_OK (0) or a negative error code; cupoly_last_error() gives the text.
+ *
+ * Data layout (same as the reference, include/cuPoly/settings.h:70-80 and
+ * src/arithmetic/polynomial.cu:206-217): a polynomial of ring degree n over a
+ * base with nres residues is cupoly_gi[nres][n/2]; coefficient j of residue r
+ * is data[r*n/2 + j].re for j < n/2 and data[r*n/2 + j - n/2].imag otherwise.
+ * The DGT domain stores X_k at position bitrev(k), exactly as the reference.
+ */
+#ifndef CUPOLY_AMD_H
+#define CUPOLY_AMD_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t re, imag; } cupoly_gi; /* GaussianInteger, settings.h:70-80 */
+
+typedef struct cupoly_engine cupoly_engine;     /* CUDAEngine + DGTEngine state */
+typedef struct cupoly_ctx cupoly_ctx;           /* Context (stream + scratch)   */
+
+/* Error codes (the reference asserts / throws std::runtime_error / exit(1)). */
+enum {
+  CUPOLY_OK = 0,
+  CUPOLY_ERR_ARG = -1,        /* null pointer, bad size or enum value          */
+  CUPOLY_ERR_UNSUPPORTED = -2,/* unsupported n, base or t (runtime_error)      */
+  CUPOLY_ERR_HIP = -3,        /* a HIP runtime call failed                     */
+  CUPOLY_ERR_STATE = -4       /* engine not initialised / wrong object         */
+};
+
+/* Bases, include/cuPoly/settings.h:37 (enum bases). */
+enum { CUPOLY_QBASE = 0, CUPOLY_BBASE = 1, CUPOLY_QBBASE = 2, CUPOLY_TBASE = 3 };
+/* DGT directions, settings.h:41 (enum dgt_direction). */
+enum { CUPOLY_FORWARD = 0, CUPOLY_INVERSE = 1 };
+/* Scalar ops, settings.h:33 (enum add_mode_t). */
+enum { CUPOLY_ADD = 0, CUPOLY_SUB = 1, CUPOLY_MUL = 2, CUPOLY_MULMUL = 3, CUPOLY_ADDADD = 4 };
+/* Sampler kinds (re-specified on Philox4x32-10; sampler.h:32-41 names). */
+enum { CUPOLY_DISCRETE_GAUSSIAN = 0, CUPOLY_UNIFORM = 5, CUPOLY_TERNARY = 6 };
+
+const char *cupoly_last_error(void);
+const char *cupoly_version(void);   /* tool/version.h */
+
+/* ---- engine: CUDAEngine::init(k, kl, M, t) / destroy ---------------------
+ * cudaengine.h:109-129.  n = ring degree (the reference's M / nphi); the
+ * engine's slot count is n/2 (CUDAEngine::N).  Q = first k primes of the 63-bit
+ * bucket, B = next kl (cudaengine.cu:1249-1289).  Unlike the reference, any
+ * t >= 2 is accepted (t = 65537 included) and several engines may coexist. */
+int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_engine **out);
+int cupoly_engine_destroy(cupoly_engine *e);
+/* CUDAEngine::get_n_residues(base), cudaengine.h:406. */
+int cupoly_get_n_residues(const cupoly_engine *e, int base);
+/* CUDAEngine::N (= n/2), CUDAEngine::RNSPrimes / RNSBPrimes / t. */
+int cupoly_engine_info(const cupoly_engine *e, int *n, int *ng, int *k, int *kl, uint64_t *t);
+int cupoly_engine_primes(const cupoly_engine *e, uint64_t *out /* k+kl */);
+/* HPS constants as computed by the engine (CUDAEngine::precompute,
+ * cudaengine.cu:1337-1511); for tests.  name: "invQqi" "Qqibi" "Qbi" "invBbi"
+ * "Bbiqi" "Bqi" "lambda" "wint" "wfrac" "twint" "twfrac"; returns count. */
+int cupoly_engine_constant(const cupoly_engine *e, const char *name, uint64_t *out, int cap);
+/* The same constants for Q = first k bucket primes, B = next kl, computed on
+ * the host only (no device, no engine); "rcp_q" gives the FP64 bits of
+ * RD(1/RN(q_j)). */
+int cupoly_hps_constant(int k, int kl, uint64_t t, const char *name, uint64_t *out, int cap);
+
+/* ---- context: Context() / ~Context(), context.cpp:19-67 ------------------
+ * stream = 0 creates a private stream (as the reference); otherwise the
+ * context uses the caller's hipStream_t. */
+int cupoly_ctx_create(cupoly_engine *e, void *stream, cupoly_ctx **out);
+int cupoly_ctx_destroy(cupoly_ctx *c);
+void *cupoly_ctx_stream(const cupoly_ctx *c);   /* Context::get_stream() */
+int cupoly_ctx_sync(const cupoly_ctx *c);
+
+/* ---- DGT: DGTEngine::execute_dgt(data, base, direction, ctx), dgt.h:141 --
+ * In place over all residues of `base`. */
+int cupoly_dgt(cupoly_ctx *c, cupoly_gi *data, int base, int direction);
+
+/* ---- pointwise (DGT domain), dgt.h:158-281 --------------------------------
+ * base selects the residue range (Q, B or QB); all on the context's stream. */
+int cupoly_add_dgt(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b, int base);
+int cupoly_sub_dgt(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b, int base);
+int cupoly_mul_dgt_gi(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b, int base);
+/* execute_mul_dgt_u64: b holds one uint64 per residue slot ([nres][n/2]). */
+int cupoly_mul_dgt_u64(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const uint64_t *b, int base);
+int cupoly_muladd_dgt(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
+                      const cupoly_gi *add, int base);
+int cupoly_double_add_dgt(cupoly_ctx *c, cupoly_gi *c1, const cupoly_gi *a1, const cupoly_gi *b1,
+                          cupoly_gi *c2, const cupoly_gi *a2, const cupoly_gi *b2, int base);
+/* execute_dr2_dgt: ct21 = ct01*ct11, ct22 = ct01*ct12 + ct02*ct11, ct23 = ct02*ct12. */
+int cupoly_dr2_dgt(cupoly_ctx *c, cupoly_gi *ct21, cupoly_gi *ct22, cupoly_gi *ct23,
+                   const cupoly_gi *ct01, const cupoly_gi *ct02, const cupoly_gi *ct11,
+                   const cupoly_gi *ct12, int base);
+
+/* ---- scalar ops, cudaengine.h:159-211 ------------------------------------
+ * execute_polynomial_negate; op_by_int (ADD/SUB/MUL by x, x reduced per
+ * residue); double_op_by_int (MULMUL: b1 = a1*x1, b2 = a2*x2; ADDADD). */
+int cupoly_negate(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, int base);
+int cupoly_op_by_int(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, uint64_t x, int op, int base);
+int cupoly_double_op_by_int(cupoly_ctx *c, cupoly_gi *b1, const cupoly_gi *a1, uint64_t x1,
+                            cupoly_gi *b2, const cupoly_gi *a2, uint64_t x2, int op, int base);
+
+/* ---- HPS, cudaengine.h:213-296 (coefficient-domain inputs) --------------- */
+/* execute_polynomial_basis_ext_Q_to_B(a, ctx): in place, Q residues in, B out
+ * (buffer sized for kl residues). */
+int cupoly_basis_ext_q_to_b(cupoly_ctx *c, cupoly_gi *a);
+/* execute_polynomial_complex_scaling(b, a_Q, a_B, ctx): b = uint64[kl][n]. */
+int cupoly_complex_scaling(cupoly_ctx *c, uint64_t *b, const cupoly_gi *a_q, const cupoly_gi *a_b);
+/* execute_polynomial_basis_ext_B_to_Q(b, a, ctx): a = uint64[kl][n] -> b Q. */
+int cupoly_basis_ext_b_to_q(cupoly_ctx *c, cupoly_gi *b, const uint64_t *a);
+/* execute_polynomial_simple_scaling(a, ctx): Q -> T in place (t a power of 2). */
+int cupoly_simple_scaling(cupoly_ctx *c, cupoly_gi *a);
+
+/* ---- poly_t-level compositions (polynomial.cu) ----------------------------
+ * DGT-domain inputs and outputs, like the reference's poly_* wrappers.       */
+/* poly_basis_extension_Q_to_B (polynomial.cu:407-457): out_b = B part. */
+int cupoly_poly_basis_ext_q_to_b(cupoly_ctx *c, cupoly_gi *out_b, const cupoly_gi *a_q);
+/* poly_complex_scaling_tDivQ (polynomial.cu:350-404); inputs are not
+ * destroyed (the reference transforms them in place). */
+int cupoly_poly_complex_scaling(cupoly_ctx *c, cupoly_gi *out_q, const cupoly_gi *a_q,
+                                const cupoly_gi *a_b);
+
+/* ---- Sampler::sample(ctx, p, kind), sampler.h:77-87 ------------------------
+ * Counter-based Philox4x32-10: the value of coefficient j of `stream` under
+ * `seed` is fixed, independent of launch shape.  Output in the DGT domain of
+ * `base` (as the reference: sample, then forward DGT). */
+int cupoly_sample(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64_t stream, int base);
+/* Same draws in the coefficient domain (no DGT). */
+int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64_t stream,
+                        int base);
+
+/* ---- batched products (no reference counterpart: the reference issues one
+ * poly_mul per product, 7 launches each, polynomial.cu:892-914) ------------
+ * out[i] = a[i] * b[i] mod (X^n + 1, q_r) for `batch` products of the engine's
+ * base-Q shape, coefficient domain in and out, [batch][k][n/2] GIs.  One fused
+ * launch: forward DGT of a and b, pointwise product, inverse DGT. */
+int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
+                         int batch);
+/* The same products one launch per op, in the reference's order
+ * (fwd a, fwd b, mul, inv) through cupoly_dgt / cupoly_mul_dgt_gi; a and b are
+ * left in the DGT domain afterwards. */
+int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi *b, int batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUPOLY_AMD_H */

@@ -1,0 +1,215 @@
+"""GPU parity: every HIP entry point against the CPU oracle on the same seeded
+inputs, bit for bit (integer results; the HPS FP64 steps too, since the oracle
+restates them with the same IEEE-754 operations)."""
+import numpy as np
+import pytest
+
+import cupoly_amd as cp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_res(P, primes, shape, seed):
+    """uniform residues, GI layout [..., nres, Ng, 2]"""
+    rng = np.random.default_rng(seed)
+    out = np.empty(tuple(shape) + (len(primes), P.Ng, 2), dtype=np.uint64)
+    for r, p in enumerate(primes):
+        out[..., r, :, :] = rng.integers(0, p, size=tuple(shape) + (P.Ng, 2), dtype=np.uint64)
+    return out
+
+
+_engines = {}
+
+
+def engine(k, kl, n, t=65537):
+    key = (k, kl, n, t)
+    if key not in _engines:
+        e = cp.CUDAEngine(k, kl, n, t)
+        _engines[key] = (e, cp.Context(e))
+    return _engines[key]
+
+
+def run_dgt(ctx, x, base, direction):
+    d = cp.to_device(x)
+    cp.execute_dgt(ctx, d, base, direction)
+    return cp.to_host(d)
+
+
+@pytest.mark.parametrize("n", [32, 64, 256, 1024, 2048, 4096, 8192, 16384, 32768, 65536])
+def test_dgt_forward_inverse_match_oracle(gpu, n):
+    k = 2 if n <= 16384 else 1
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    for base in (cp.QBase, cp.BBase, cp.QBBase):
+        x = rand_res(P, P.base_primes(base), (), n + base)
+        f = run_dgt(ctx, x, base, cp.FORWARD)
+        assert np.array_equal(f, O.dgt(P, x, base, 0)), ("forward", n, base)
+        g = run_dgt(ctx, f, base, cp.INVERSE)
+        assert np.array_equal(g, x), ("inverse", n, base)
+
+
+@pytest.mark.parametrize("n,k,batch", [(32, 1, 5), (2048, 2, 3), (4096, 3, 2), (16384, 4, 2),
+                                       (32768, 3, 1), (65536, 1, 1)])
+def test_polymul_fused_and_plain_match_oracle(gpu, n, k, batch):
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 1 + n)
+    b = rand_res(P, P.q, (batch,), 2 + n)
+    want = O.polymul_batch(P, a, b)
+    da, db = cp.to_device(a), cp.to_device(b)
+    out = cp.empty(a.shape)
+    cp.polymul_batch(ctx, out, da, db)
+    assert np.array_equal(cp.to_host(out), want), "fused"
+    assert np.array_equal(cp.to_host(da), a) and np.array_equal(cp.to_host(db), b)
+    out2 = cp.empty(a.shape)
+    cp.polymul_plain(ctx, out2, da, db)
+    assert np.array_equal(cp.to_host(out2), want), "plain"
+
+
+def test_config1_golden_fixture(gpu):
+    import os
+    g = np.load(os.path.join(O.REPO, "tests", "golden", "cfg1_polymul.npz"))
+    e, ctx = engine(2, 3, 2048, 256)
+    a = cp.to_device(O.fold(g["a"], 1024)[None])
+    b = cp.to_device(O.fold(g["b"], 1024)[None])
+    out = cp.empty(a.shape)
+    cp.polymul_batch(ctx, out, a, b)
+    assert np.array_equal(O.unfold(cp.to_host(out)[0]), g["c"])
+
+
+@pytest.mark.parametrize("base", [0, 1, 2])
+def test_pointwise_ops_match_oracle(gpu, base):
+    n = 2048
+    e, ctx = engine(3, 4, n)
+    P = O.Params(3, 4, n, 65537)
+    pr = P.base_primes(base)
+    a, b, c = (rand_res(P, pr, (), s) for s in (11, 12, 13))
+    da, db, dc = cp.to_device(a), cp.to_device(b), cp.to_device(c)
+    out = cp.empty(a.shape)
+    for op, fn, args in [(0, cp.execute_add_dgt, (da, db)), (1, cp.execute_sub_dgt, (da, db)),
+                         (2, cp.execute_mul_dgt_gi, (da, db)),
+                         (3, cp.execute_muladd_dgt, (da, db, dc))]:
+        fn(ctx, out, *args, base=base)
+        assert np.array_equal(cp.to_host(out), O.pointwise(P, a, b, c, base=base, op=op)), op
+    # dr2: ct21 = a0 b0, ct22 = a0 b1 + a1 b0, ct23 = a1 b1
+    a1, b1 = rand_res(P, pr, (), 14), rand_res(P, pr, (), 15)
+    o1, o2, o3 = cp.empty(a.shape), cp.empty(a.shape), cp.empty(a.shape)
+    cp.execute_dr2_dgt(ctx, o1, o2, o3, da, cp.to_device(a1), db, cp.to_device(b1), base=base)
+    assert np.array_equal(cp.to_host(o1), O.pointwise(P, a, b, base=base, op=2))
+    cross = O.pointwise(P, O.pointwise(P, a, b1, base=base, op=2),
+                        O.pointwise(P, a1, b, base=base, op=2), base=base, op=0)
+    assert np.array_equal(cp.to_host(o2), cross)
+    assert np.array_equal(cp.to_host(o3), O.pointwise(P, a1, b1, base=base, op=2))
+    # negate, op_by_int
+    cp.execute_polynomial_negate(ctx, out, da, base=base)
+    want = np.stack([np.where(a[r] == 0, 0, np.uint64(p) - a[r]) for r, p in enumerate(pr)])
+    assert np.array_equal(cp.to_host(out), want)
+    x = 123456789
+    for op in (cp.ADD, cp.SUB, cp.MUL):
+        cp.execute_polynomial_op_by_int(ctx, out, da, x, op, base=base)
+        const = np.zeros_like(a)
+        const[:, :, 0] = x
+        if op == cp.MUL:
+            ref = O.pointwise(P, a, const, base=base, op=2)
+        else:
+            ref = O.pointwise(P, a, const, base=base, op=0 if op == cp.ADD else 1)
+        assert np.array_equal(cp.to_host(out), ref), op
+    # mul by a uint64 per slot
+    u = np.random.default_rng(9).integers(0, 1 << 62, size=(len(pr), P.Ng), dtype=np.uint64)
+    cp.execute_mul_dgt_u64(ctx, out, da, cp.to_device(u), base=base)
+    g = np.zeros_like(a)
+    g[..., 0] = u
+    uu = np.stack([g[r] % np.uint64(p) for r, p in enumerate(pr)])
+    assert np.array_equal(cp.to_host(out), O.pointwise(P, a, uu, base=base, op=2))
+
+
+@pytest.mark.parametrize("k,n,t", [(1, 1024, 256), (2, 2048, 256), (4, 16384, 65537),
+                                   (8, 4096, 65537)])
+def test_hps_kernels_match_oracle(gpu, k, n, t):
+    kl = k + 1
+    e, ctx = engine(k, kl, n, t)
+    P = O.Params(k, kl, n, t)
+    xq = rand_res(P, P.q, (), 40 + k)
+    # Q -> B (in place in a buffer of max(k, kl) residues)
+    buf = np.zeros((max(k, kl), P.Ng, 2), dtype=np.uint64)
+    buf[:k] = xq
+    d = cp.to_device(buf)
+    cp.execute_polynomial_basis_ext_Q_to_B(ctx, d)
+    assert np.array_equal(cp.to_host(d)[:kl], O.basis_ext_q_to_b(P, xq))
+    # complex scaling from QB residues of random x in [0, QB)
+    xb = rand_res(P, P.b, (), 50 + k)
+    out = cp.empty((kl, n))
+    cp.execute_polynomial_complex_scaling(ctx, out, cp.to_device(xq), cp.to_device(xb))
+    cs = O.complex_scaling(P, xq, xb)
+    assert np.array_equal(cp.to_host(out), cs)
+    # B -> Q on the complex-scaling output
+    oq = cp.empty((k, P.Ng, 2))
+    cp.execute_polynomial_basis_ext_B_to_Q(ctx, oq, out)
+    assert np.array_equal(cp.to_host(oq), O.basis_ext_b_to_q(P, cs))
+    if t & (t - 1) == 0:
+        d = cp.to_device(xq)
+        cp.execute_polynomial_simple_scaling(ctx, d)
+        assert np.array_equal(cp.to_host(d)[0], O.simple_scaling(P, xq))
+
+
+def test_hps_poly_level_compositions(gpu):
+    """poly_basis_extension_Q_to_B and poly_complex_scaling_tDivQ on DGT-domain
+    operands, k = 4, n = 2^14, t = 65537 (the BFV-size shape)."""
+    k, kl, n, t = 4, 5, 16384, 65537
+    e, ctx = engine(k, kl, n, t)
+    P = O.Params(k, kl, n, t)
+    aq = rand_res(P, P.q, (), 61)
+    aq_dgt = O.dgt(P, aq, 0, 0)
+    ob = cp.empty((kl, P.Ng, 2))
+    cp.poly_basis_extension_Q_to_B(ctx, ob, cp.to_device(aq_dgt))
+    want_b = O.dgt(P, O.basis_ext_q_to_b(P, aq), 1, 0)
+    assert np.array_equal(cp.to_host(ob), want_b)
+    oq = cp.empty((k, P.Ng, 2))
+    dq = cp.to_device(aq_dgt)
+    cp.poly_complex_scaling_tDivQ(ctx, oq, dq, ob)
+    want_q = O.dgt(P, O.basis_ext_b_to_q(P, O.complex_scaling(P, aq, O.basis_ext_q_to_b(P, aq))),
+                   0, 0)
+    assert np.array_equal(cp.to_host(oq), want_q)
+    assert np.array_equal(cp.to_host(dq), aq_dgt)  # inputs preserved
+
+
+@pytest.mark.parametrize("kind", [cp.UNIFORM, cp.TERNARY, cp.DISCRETE_GAUSSIAN])
+def test_sampler_matches_oracle_counters(gpu, kind):
+    n, k = 4096, 3
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    out = cp.empty((k, P.Ng, 2))
+    cp.Sampler.sample(ctx, out, kind, 777, 5, base=cp.QBase, coeff_domain=True)
+    if kind == cp.UNIFORM:
+        want = O.sample_uniform(P.q, n, 777, 5)
+    else:
+        vals = O.sample_small(n, 2 if kind == cp.TERNARY else 3, 777, 5)
+        want = O.small_to_residues(vals, P.q)
+    assert np.array_equal(O.unfold(cp.to_host(out)), want)
+    # DGT-domain variant = forward DGT of the same draws
+    cp.Sampler.sample(ctx, out, kind, 777, 5, base=cp.QBase)
+    assert np.array_equal(cp.to_host(out), O.dgt(P, O.fold(want, P.Ng), 0, 0))
+
+
+def test_large_batch_properties(gpu):
+    """n = 2^14, k = 4, 64 products: fused == plain, and a * 1 = a."""
+    import torch
+    k, n, batch = 4, 16384, 64
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 71)
+    b = rand_res(P, P.q, (batch,), 72)
+    da, db = cp.to_device(a), cp.to_device(b)
+    o1, o2 = cp.empty(a.shape), cp.empty(a.shape)
+    cp.polymul_batch(ctx, o1, da, db)
+    cp.polymul_plain(ctx, o2, da.clone(), db.clone())
+    assert torch.equal(o1, o2)
+    one = np.zeros_like(b)
+    one[:, :, 0, 0] = 1
+    cp.polymul_batch(ctx, o2, da, cp.to_device(one))
+    assert torch.equal(o2, da)
+    # spot-check three products against the oracle
+    h = cp.to_host(o1)
+    for i in (0, 31, 63):
+        assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
