@@ -56,7 +56,25 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_facade_test(verbose)
     return LIB
+
+
+def build_facade_test(verbose=False):
+    """tests/cpp/facade_test: a C++ caller of include/cupoly_amd.hpp."""
+    repo = os.path.dirname(HERE)
+    src = os.path.join(repo, "tests", "cpp", "facade_test.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(repo, "tests", "cpp", "facade_test")
+    cmd = [HIPCC, "-std=c++17", "-O2", src, "-o", out, "-L" + HERE, "-lcupoly_amd",
+           "-Wl,-rpath,$ORIGIN/../../cupoly_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade test build failed:\n%s%s" % (r.stdout, r.stderr))
+    if verbose:
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,378 @@
+// cupoly_amd.hpp - C++ facade with the reference's host API, over the C ABI.
+//
+// A caller written against cuPoly's device-side API (include/cuPoly/cuda/
+// cudaengine.h, dgt.h, sampler.h, arithmetic/context.h, arithmetic/
+// polynomial.h) keeps its names and argument order:
+//
+//   CUDAEngine::init(k, kl, M, t);  Context ctx;  poly_t a, b, c;
+//   Sampler::sample(&ctx, &a, UNIFORM);  poly_mul(&ctx, &c, &a, &b);
+//   DGTEngine::execute_dgt(c.d_coefs, QBase, INVERSE, &ctx);
+//
+// Differences a caller sees:
+//   * errors throw std::runtime_error with the library's message instead of
+//     asserting or calling exit(1) (settings.h:144-152);
+//   * streams are hipStream_t; the DGTEngine pointwise calls that take a
+//     stream run on a context bound to that stream;
+//   * poly_t keeps residues, not NTL ZZ coefficients: host data enters and
+//     leaves through poly_import_residues / poly_export_residues /
+//     poly_get_residues (the reference's own residue wire format,
+//     polynomial.cu:1291-1393);
+//   * Sampler draws are counter-based (Philox4x32-10): Sampler::sample takes
+//     an optional (seed, stream) and otherwise uses a per-process counter.
+//
+// Header-only; link with libcupoly_amd.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cupoly_amd.h"
+
+typedef cupoly_gi GaussianInteger;
+enum bases { QBase = CUPOLY_QBASE, BBase = CUPOLY_BBASE, QBBase = CUPOLY_QBBASE, TBase = CUPOLY_TBASE };
+enum dgt_direction { FORWARD = CUPOLY_FORWARD, INVERSE = CUPOLY_INVERSE };
+enum add_mode_t { ADD = CUPOLY_ADD, SUB = CUPOLY_SUB, MUL = CUPOLY_MUL, MULMUL = CUPOLY_MULMUL,
+                  ADDADD = CUPOLY_ADDADD };
+enum pdistributions { DISCRETE_GAUSSIAN = CUPOLY_DISCRETE_GAUSSIAN, UNIFORM = CUPOLY_UNIFORM,
+                      TERNARY = CUPOLY_TERNARY };
+enum poly_states { HOSTSTATE, RNSSTATE, BOTHSTATE };
+
+namespace cupoly_detail {
+inline void check(int rc) {
+  if (rc < 0) throw std::runtime_error(cupoly_last_error());
+}
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace cupoly_detail
+
+// CUDAEngine: the reference's process-wide engine (cudaengine.h:65-407),
+// backed by one cupoly_engine.
+class CUDAEngine {
+ public:
+  inline static cupoly_engine *engine = nullptr;
+  inline static int N = 0;  // n / 2 Gaussian slots
+  inline static uint64_t t = 0;
+  inline static bool is_init = false;
+  inline static std::vector<uint64_t> RNSPrimes, RNSBPrimes;
+
+  static void init(int k, int kl, int M, uint64_t plaintext) {
+    if (is_init) destroy();
+    cupoly_detail::check(cupoly_engine_create(k, kl, M, plaintext, &engine));
+    N = M / 2;
+    t = plaintext;
+    std::vector<uint64_t> all(k + kl);
+    cupoly_detail::check(cupoly_engine_primes(engine, all.data()));
+    RNSPrimes.assign(all.begin(), all.begin() + k);
+    RNSBPrimes.assign(all.begin() + k, all.end());
+    is_init = true;
+  }
+  static void init(int k, int M, uint64_t plaintext) { init(k, k + 1, M, plaintext); }
+  static void destroy();
+  static int get_n_residues(int base) {
+    int r = cupoly_get_n_residues(engine, base);
+    cupoly_detail::check(r);
+    return r;
+  }
+};
+
+// Context: stream + scratch (context.h:36-109).
+class Context {
+ public:
+  cupoly_ctx *ctx = nullptr;
+  explicit Context(hipStream_t stream = nullptr) {
+    if (!CUDAEngine::is_init) throw std::runtime_error("Context: CUDAEngine not initialised");
+    cupoly_detail::check(cupoly_ctx_create(CUDAEngine::engine, (void *)stream, &ctx));
+  }
+  ~Context() {
+    if (ctx) cupoly_ctx_destroy(ctx);
+  }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  hipStream_t get_stream() { return (hipStream_t)cupoly_ctx_stream(ctx); }
+  void sync() { cupoly_detail::check(cupoly_ctx_sync(ctx)); }
+};
+
+namespace cupoly_detail {
+inline std::map<hipStream_t, Context *> &stream_contexts() {
+  static std::map<hipStream_t, Context *> m;
+  return m;
+}
+inline Context *for_stream(hipStream_t s) {
+  auto &m = stream_contexts();
+  auto it = m.find(s);
+  if (it != m.end()) return it->second;
+  Context *c = new Context(s);
+  m[s] = c;
+  return c;
+}
+}  // namespace cupoly_detail
+
+inline void CUDAEngine::destroy() {
+  for (auto &kv : cupoly_detail::stream_contexts()) delete kv.second;
+  cupoly_detail::stream_contexts().clear();
+  if (engine) cupoly_engine_destroy(engine);
+  engine = nullptr;
+  is_init = false;
+  N = 0;
+  RNSPrimes.clear();
+  RNSBPrimes.clear();
+}
+
+// DGTEngine (dgt.h:90-282).
+class DGTEngine {
+ public:
+  static void execute_dgt(GaussianInteger *data, int base, int direction, Context *ctx) {
+    cupoly_detail::check(cupoly_dgt(ctx->ctx, data, base, direction));
+  }
+  static void execute_add_dgt(GaussianInteger *c, const GaussianInteger *a,
+                              const GaussianInteger *b, int base, hipStream_t s) {
+    cupoly_detail::check(cupoly_add_dgt(cupoly_detail::for_stream(s)->ctx, c, a, b, base));
+  }
+  static void execute_sub_dgt(GaussianInteger *c, const GaussianInteger *a,
+                              const GaussianInteger *b, int base, hipStream_t s) {
+    cupoly_detail::check(cupoly_sub_dgt(cupoly_detail::for_stream(s)->ctx, c, a, b, base));
+  }
+  static void execute_double_add_dgt(GaussianInteger *c1, const GaussianInteger *a1,
+                                     const GaussianInteger *b1, GaussianInteger *c2,
+                                     const GaussianInteger *a2, const GaussianInteger *b2,
+                                     int base, hipStream_t s) {
+    cupoly_detail::check(
+        cupoly_double_add_dgt(cupoly_detail::for_stream(s)->ctx, c1, a1, b1, c2, a2, b2, base));
+  }
+  static void execute_dr2_dgt(GaussianInteger *ct21, GaussianInteger *ct22, GaussianInteger *ct23,
+                              const GaussianInteger *ct01, const GaussianInteger *ct02,
+                              const GaussianInteger *ct11, const GaussianInteger *ct12, int base,
+                              hipStream_t s) {
+    cupoly_detail::check(cupoly_dr2_dgt(cupoly_detail::for_stream(s)->ctx, ct21, ct22, ct23, ct01,
+                                        ct02, ct11, ct12, base));
+  }
+  static void execute_mul_dgt_gi(GaussianInteger *c, const GaussianInteger *a,
+                                 const GaussianInteger *b, int base, hipStream_t s) {
+    cupoly_detail::check(cupoly_mul_dgt_gi(cupoly_detail::for_stream(s)->ctx, c, a, b, base));
+  }
+  static void execute_mul_dgt_u64(GaussianInteger *c, const GaussianInteger *a, const uint64_t *b,
+                                  int base, hipStream_t s) {
+    cupoly_detail::check(cupoly_mul_dgt_u64(cupoly_detail::for_stream(s)->ctx, c, a, b, base));
+  }
+  static void execute_muladd_dgt(GaussianInteger *d, const GaussianInteger *a,
+                                 const GaussianInteger *b, const GaussianInteger *c, int base,
+                                 hipStream_t s) {
+    cupoly_detail::check(cupoly_muladd_dgt(cupoly_detail::for_stream(s)->ctx, d, a, b, c, base));
+  }
+};
+
+// poly_t (polynomial.h:55-66), residues only.
+struct poly_t {
+  bool init = false;
+  GaussianInteger *d_coefs = nullptr;
+  int status = BOTHSTATE;
+  int base = QBase;
+};
+
+inline size_t poly_get_residues_size(int base) {
+  return (size_t)CUDAEngine::N * CUDAEngine::get_n_residues(base) * sizeof(GaussianInteger);
+}
+
+inline void poly_init(Context *ctx, poly_t *a, int base = QBase) {
+  if (a->init) return;  // idempotent (polynomial.cu:51-52)
+  a->base = base;
+  cupoly_detail::hip_check(hipMalloc((void **)&a->d_coefs, poly_get_residues_size(base)),
+                           "poly_init");
+  cupoly_detail::hip_check(
+      hipMemsetAsync(a->d_coefs, 0, poly_get_residues_size(base), ctx->get_stream()), "poly_init");
+  a->init = true;
+  a->status = BOTHSTATE;
+}
+
+inline void poly_free(Context *ctx, poly_t *a) {
+  if (!a || !a->init) return;
+  ctx->sync();
+  (void)hipFree(a->d_coefs);
+  a->d_coefs = nullptr;
+  a->init = false;
+}
+
+inline void poly_clear(Context *ctx, poly_t *a) {
+  poly_init(ctx, a, a->base);
+  cupoly_detail::hip_check(
+      hipMemsetAsync(a->d_coefs, 0, poly_get_residues_size(a->base), ctx->get_stream()),
+      "poly_clear");
+}
+
+inline void poly_copy(Context *ctx, poly_t *b, poly_t *a) {
+  poly_init(ctx, b, a->base);
+  cupoly_detail::hip_check(hipMemcpyAsync(b->d_coefs, a->d_coefs, poly_get_residues_size(a->base),
+                                          hipMemcpyDeviceToDevice, ctx->get_stream()),
+                           "poly_copy");
+  b->status = a->status;
+}
+
+inline void poly_add(Context *ctx, poly_t *c, poly_t *a, poly_t *b) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_add_dgt(ctx->ctx, c->d_coefs, a->d_coefs, b->d_coefs, a->base));
+  c->status = RNSSTATE;
+}
+inline void poly_sub(Context *ctx, poly_t *c, poly_t *a, poly_t *b) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_sub_dgt(ctx->ctx, c->d_coefs, a->d_coefs, b->d_coefs, a->base));
+  c->status = RNSSTATE;
+}
+inline void poly_double_add(Context *ctx, poly_t *c1, poly_t *a1, poly_t *b1, poly_t *c2,
+                            poly_t *a2, poly_t *b2) {
+  poly_add(ctx, c1, a1, b1);
+  poly_add(ctx, c2, a2, b2);
+}
+inline void poly_mul(Context *ctx, poly_t *c, poly_t *a, poly_t *b) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_mul_dgt_gi(ctx->ctx, c->d_coefs, a->d_coefs, b->d_coefs, a->base));
+  c->status = RNSSTATE;
+}
+inline void poly_mul_add(Context *ctx, poly_t *d, poly_t *a, poly_t *b, poly_t *c) {
+  poly_init(ctx, d, a->base);
+  cupoly_detail::check(
+      cupoly_muladd_dgt(ctx->ctx, d->d_coefs, a->d_coefs, b->d_coefs, c->d_coefs, a->base));
+  d->status = RNSSTATE;
+}
+inline void poly_dr2(Context *ctx, poly_t *ct21, poly_t *ct22, poly_t *ct23, poly_t *ct01,
+                     poly_t *ct02, poly_t *ct11, poly_t *ct12) {
+  for (poly_t *o : {ct21, ct22, ct23}) poly_init(ctx, o, ct01->base);
+  cupoly_detail::check(cupoly_dr2_dgt(ctx->ctx, ct21->d_coefs, ct22->d_coefs, ct23->d_coefs,
+                                      ct01->d_coefs, ct02->d_coefs, ct11->d_coefs, ct12->d_coefs,
+                                      ct01->base));
+}
+inline void poly_mul_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, MUL, a->base));
+}
+inline void poly_add_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, ADD, a->base));
+}
+inline void poly_sub_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
+  poly_init(ctx, c, a->base);
+  cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, SUB, a->base));
+}
+inline void poly_double_mul_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2, poly_t *a2,
+                                uint64_t x1, uint64_t x2) {
+  poly_init(ctx, c1, a1->base);
+  poly_init(ctx, c2, a2->base);
+  cupoly_detail::check(cupoly_double_op_by_int(ctx->ctx, c1->d_coefs, a1->d_coefs, x1,
+                                               c2->d_coefs, a2->d_coefs, x2, MULMUL, a1->base));
+}
+inline void poly_double_add_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2, poly_t *a2,
+                                uint64_t x1, uint64_t x2) {
+  poly_init(ctx, c1, a1->base);
+  poly_init(ctx, c2, a2->base);
+  cupoly_detail::check(cupoly_double_op_by_int(ctx->ctx, c1->d_coefs, a1->d_coefs, x1,
+                                               c2->d_coefs, a2->d_coefs, x2, ADDADD, a1->base));
+}
+inline void poly_negate(Context *ctx, poly_t *a) {
+  cupoly_detail::check(cupoly_negate(ctx->ctx, a->d_coefs, a->d_coefs, a->base));
+}
+inline void poly_basis_extension_Q_to_B(Context *ctx, poly_t *a_B, poly_t *a_Q) {
+  poly_init(ctx, a_B, BBase);
+  cupoly_detail::check(cupoly_poly_basis_ext_q_to_b(ctx->ctx, a_B->d_coefs, a_Q->d_coefs));
+  a_B->status = RNSSTATE;
+}
+inline void poly_complex_scaling_tDivQ(Context *ctx, poly_t *b, poly_t *a_Q, poly_t *a_B) {
+  poly_init(ctx, b, QBase);
+  cupoly_detail::check(
+      cupoly_poly_complex_scaling(ctx->ctx, b->d_coefs, a_Q->d_coefs, a_B->d_coefs));
+  b->status = RNSSTATE;
+}
+
+// Coefficient-domain residues of a, [nres][2N] (re half then imag half per
+// residue), as poly_export_residues lays them out (polynomial.cu:1321-1330).
+inline std::vector<uint64_t> poly_get_residues(Context *ctx, poly_t *a) {
+  const int nres = CUDAEngine::get_n_residues(a->base), N = CUDAEngine::N;
+  GaussianInteger *tmp = nullptr;
+  const size_t bytes = poly_get_residues_size(a->base);
+  cupoly_detail::hip_check(hipMalloc((void **)&tmp, bytes), "poly_get_residues");
+  cupoly_detail::hip_check(
+      hipMemcpyAsync(tmp, a->d_coefs, bytes, hipMemcpyDeviceToDevice, ctx->get_stream()),
+      "poly_get_residues");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, tmp, a->base, INVERSE));
+  std::vector<GaussianInteger> h((size_t)nres * N);
+  cupoly_detail::hip_check(
+      hipMemcpyAsync(h.data(), tmp, bytes, hipMemcpyDeviceToHost, ctx->get_stream()),
+      "poly_get_residues");
+  ctx->sync();
+  (void)hipFree(tmp);
+  std::vector<uint64_t> out((size_t)nres * 2 * N);
+  for (int r = 0; r < nres; r++)
+    for (int c = 0; c < N; c++) {
+      out[(size_t)r * 2 * N + c] = h[(size_t)r * N + c].re;
+      out[(size_t)r * 2 * N + N + c] = h[(size_t)r * N + c].imag;
+    }
+  return out;
+}
+
+inline std::string poly_export_residues(Context *ctx, poly_t *a) {
+  std::vector<uint64_t> v = poly_get_residues(ctx, a);
+  std::ostringstream os;
+  for (size_t i = 0; i < v.size(); i++) os << (i ? " " : "") << (int64_t)v[i];
+  return os.str();
+}
+
+// Residues (coefficient domain, [nres][2N]) -> a new poly_t in the DGT domain.
+inline poly_t *poly_import_residues(Context *ctx, const std::vector<uint64_t> &v, int base = QBase) {
+  const int nres = CUDAEngine::get_n_residues(base), N = CUDAEngine::N;
+  std::vector<GaussianInteger> h((size_t)nres * N, GaussianInteger{0, 0});
+  for (size_t i = 0; i < v.size() && i < (size_t)nres * 2 * N; i++) {
+    const int cid = (int)(i % (2 * N)), rid = (int)(i / (2 * N));
+    if (cid < N)
+      h[(size_t)rid * N + cid].re = v[i];
+    else
+      h[(size_t)rid * N + cid - N].imag = v[i];
+  }
+  poly_t *a = new poly_t;
+  poly_init(ctx, a, base);
+  cupoly_detail::hip_check(hipMemcpyAsync(a->d_coefs, h.data(), poly_get_residues_size(base),
+                                          hipMemcpyHostToDevice, ctx->get_stream()),
+                           "poly_import_residues");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, a->d_coefs, base, FORWARD));
+  ctx->sync();
+  a->status = RNSSTATE;
+  return a;
+}
+
+inline poly_t *poly_import_residues(Context *ctx, const std::string &s, int base = QBase) {
+  std::istringstream is(s);
+  std::vector<uint64_t> v;
+  int64_t x;
+  while (is >> x) v.push_back((uint64_t)x);
+  return poly_import_residues(ctx, v, base);
+}
+
+inline bool poly_are_equal(Context *ctx, poly_t *a, poly_t *b) {
+  if (a->base != b->base) return false;
+  return poly_get_residues(ctx, a) == poly_get_residues(ctx, b);
+}
+
+// Sampler (sampler.h:49-99), counter-based.
+class Sampler {
+ public:
+  inline static uint64_t seed = 42;  // SEED, sampler.h:44
+  inline static uint64_t next_stream = 0;
+  static void init(Context *) {}
+  static void reset(Context *) { next_stream = 0; }
+  static void destroy() {}
+  static void sample(Context *ctx, poly_t *p, int kind) { sample(ctx, p, kind, seed, next_stream++); }
+  static void sample(Context *ctx, poly_t *p, int kind, uint64_t s, uint64_t stream) {
+    poly_init(ctx, p, p->base);
+    cupoly_detail::check(cupoly_sample(ctx->ctx, p->d_coefs, kind, s, stream, p->base));
+    p->status = RNSSTATE;
+    p->init = true;
+  }
+  static poly_t *sample(Context *ctx, int kind) {
+    poly_t *p = new poly_t;
+    sample(ctx, p, kind);
+    return p;
+  }
+};
