@@ -31,11 +31,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=4)
     ap.add_argument("--plain-batch", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--hm-batch", type=int, default=128,
+                    help="ciphertext pairs per batch for the BFV multiplication core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -111,6 +113,45 @@ def main():
     assert torch.equal(pout, out[:PB]), "plain and batched results differ"
     print("bench: plain done %.0f poly-mul/s" % plain_rate, file=sys.stderr, flush=True)
 
+    # BFV multiplication core (tensor product + HPS Q->B, t/Q, B->Q), t = 65537
+    HB = args.hm_batch
+    hm = None
+    if HB > 0:
+        ct1 = cp.empty((HB, 2, k, ng, 2))
+        ct2 = cp.empty((HB, 2, k, ng, 2))
+        for i in range(HB):
+            for j in range(2):
+                cp.Sampler.sample(ctx, ct1[i, j], cp.UNIFORM, 99, 4 * i + j, coeff_domain=True)
+                cp.Sampler.sample(ctx, ct2[i, j], cp.UNIFORM, 99, 4 * i + 2 + j, coeff_domain=True)
+        hout = cp.empty((HB, 3, k, ng, 2))
+        cp.hom_mul_batch(ctx, hout, ct1, ct2)  # warm-up (allocates the workspace)
+        torch.cuda.synchronize()
+        hreps = 5
+        t3 = time.perf_counter()
+        for _ in range(hreps):
+            cp.hom_mul_batch(ctx, hout, ct1, ct2)
+        torch.cuda.synchronize()
+        hm_s = (time.perf_counter() - t3) / hreps
+        hm_bytes = 56 * k * n * HB
+        hm = {"value": HB / hm_s, "unit": "hom-mults/s", "batch": HB, "ms_per_batch": hm_s * 1e3,
+              "workload": "ciphertext tensor product + HPS Q->B (kl=k+1), t/Q scale-and-round "
+                          "(t=65537), B->Q; coefficient domain in/out",
+              "algorithmic_bytes": hm_bytes, "achieved_GBs": hm_bytes / hm_s / 1e9,
+              "hbm_frac": hm_bytes / hm_s / 1e9 / HBM_PEAK_GBS}
+        del ct1, ct2, hout
+        print("bench: hom-mul done %.0f hom-mults/s" % (HB / hm_s), file=sys.stderr, flush=True)
+
+    # HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    # same configuration (FETCH_SIZE / WRITE_SIZE scaled by a copy of known size)
+    traffic = None
+    try:
+        with open(os.path.join(REPO, "profiles", "polymul_fused_n16384_k4.json")) as f:
+            pm = json.load(f)
+        if (pm["n"], pm["k"], pm["batch"]) == (n, k, B):
+            traffic = pm["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -146,10 +187,11 @@ def main():
                   "launches_per_product": 4},
         "speedup_vs_plain": value / plain_rate,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_polymul_fused<13,1>", "avg_launch_ms": avg_launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
+        "hom_mul": hm,
     }
     print(json.dumps(line), flush=True)
 

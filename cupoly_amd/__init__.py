@@ -66,6 +66,7 @@ SIGNATURES = {
     "cupoly_sample_coeff": ([_vp, _vp, _int, _u64, _u64, _int], _int),
     "cupoly_polymul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_polymul_plain": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
 }
 
 _lib = None
@@ -320,6 +321,17 @@ def polymul_plain(ctx, out, a, b):
     for t in (out, a, b):
         _need(t, e.k, e.N, batch)
     _check(lib().cupoly_polymul_plain(ctx.handle, _ptr(out), _ptr(a), _ptr(b), batch))
+
+
+def hom_mul_batch(ctx, out, ct1, ct2):
+    """BFV multiplication core for pairs of ciphertexts: ct1, ct2 [batch, 2, k, N, 2]
+    base Q coefficient domain -> out [batch, 3, k, N, 2] = round(t/Q * tensor)."""
+    e = ctx.engine
+    batch = ct1.shape[0]
+    for t, m in ((ct1, 2), (ct2, 2), (out, 3)):
+        if tuple(t.shape) != (batch, m, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: hom_mul_batch operand shape %s" % (tuple(t.shape),))
+    _check(lib().cupoly_hom_mul_batch(ctx.handle, _ptr(out), _ptr(ct1), _ptr(ct2), batch))
 
 
 # ---- host <-> device helpers ------------------------------------------------

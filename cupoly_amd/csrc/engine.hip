@@ -350,6 +350,7 @@ extern "C" int cupoly_ctx_destroy(cupoly_ctx *c) {
   if (c->d_aux) (void)hipFree(c->d_aux);
   if (c->d_tmp) (void)hipFree(c->d_tmp);
   if (c->d_tmp2) (void)hipFree(c->d_tmp2);
+  if (c->d_ws) (void)hipFree(c->d_ws);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUPOLY_OK;

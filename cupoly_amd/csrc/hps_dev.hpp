@@ -1,0 +1,170 @@
+// Per-slot HPS steps shared by the single-polynomial kernels (hps.hip) and
+// the batched homomorphic-multiply kernels (homomul.hip).  A thread owns one
+// Gaussian slot c: coefficients c and c + n/2 of every residue.
+//
+// Floating point is reproduced operation for operation from the reference
+// (src/cuda/cudaengine.cu:382-440, 513-534, 688-703): see hps.hip.
+#pragma once
+#include <math.h>
+
+#include "internal.hpp"
+
+extern "C" __device__ double __ocml_fma_rtn_f64(double, double, double);
+
+// The reference's addmod / submod macros (include/cuPoly/cuda/dgt.h:73-74):
+// one conditional correction on 64-bit wrap-around words, kept verbatim for
+// the additions that may see unreduced operands in the complex scaling.
+__device__ __forceinline__ uint64_t addmod_ref(uint64_t a, uint64_t b, uint64_t p) {
+  const uint64_t s = a + b;
+  return s >= p ? s - p : s;
+}
+__device__ __forceinline__ uint64_t submod_ref(uint64_t a, uint64_t b, uint64_t p) {
+  return a < b ? a + (p - b) : a - b;
+}
+
+// |value - llrint(value)| <= 1e-9 ? llrint(value) : floor(value)  (cudaengine.cu:530-531)
+__device__ __forceinline__ uint64_t snap(double value) {
+  const long long rn = llrint(value);
+  const double diff = fabs(value - (double)rn);
+  return diff <= 1e-9 ? (uint64_t)rn : (uint64_t)floor(value);
+}
+
+__device__ __forceinline__ uint64_t shoup2(uint64_t a, const uint64_t (&w)[2], uint64_t p) {
+  return cp_shoup(a, w[0], w[1], p);
+}
+
+// floor(a * b / 2^64) for a 128-bit a = (alo, ahi): (lo, hi) words
+// (the reference's __umul128hi, cudaengine.cu:313-320).
+__device__ __forceinline__ void umul128hi(uint64_t alo, uint64_t ahi, uint64_t b, uint64_t &lo,
+                                          uint64_t &hi) {
+  const cp_u128 x = (cp_u128)alo * b;
+  const cp_u128 y = (cp_u128)ahi * b + (uint64_t)(x >> 64);
+  lo = (uint64_t)y;
+  hi = (uint64_t)(y >> 64);
+}
+
+// Q -> B for slot c: in[j*qs + c] (j < k) -> out[i*bs + c] (i < kl).  All
+// inputs are read before any output is written, so in == out is allowed.
+__device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_gi *out, size_t bs,
+                                                int c, const cp_hps *__restrict__ H) {
+  const int k = H->k, kl = H->kl;
+  uint64_t ar[CP_MAX_BASE], ai[CP_MAX_BASE];
+  double vr = 0.0, vi = 0.0;
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      const uint64_t q = H->q[j].p;
+      const cp_gi x = in[(size_t)j * qs + c];
+      ar[j] = shoup2(x.re, H->invQqi[j], q);
+      ai[j] = shoup2(x.imag, H->invQqi[j], q);
+    }
+  }
+  // v = RD(aux_j * RD(1/q_j) + v) in ascending j (cudaengine.cu:526-527)
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      const double r = H->rcp_q[j];
+      vr = __ocml_fma_rtn_f64((double)ar[j], r, vr);
+      vi = __ocml_fma_rtn_f64((double)ai[j], r, vi);
+    }
+  }
+  const uint64_t v_re = snap(vr), v_im = snap(vi);
+  for (int i = 0; i < kl; i++) {
+    const uint64_t b = H->b[i].p;
+    uint64_t sr = 0, si = 0;
+#pragma unroll
+    for (int j = 0; j < CP_MAX_BASE; j++) {
+      if (j < k) {
+        sr = addmod_ref(sr, shoup2(ar[j], H->Qqibi[i][j], b), b);
+        si = addmod_ref(si, shoup2(ai[j], H->Qqibi[i][j], b), b);
+      }
+    }
+    sr = submod_ref(sr, shoup2(v_re, H->Qbi[i], b), b);
+    si = submod_ref(si, shoup2(v_im, H->Qbi[i], b), b);
+    cp_gi o;
+    o.re = sr;
+    o.imag = si;
+    out[(size_t)i * bs + c] = o;
+  }
+}
+
+// Complex scaling t/Q for slot c: aq[j*qs + c] (j < k), ab[i*bs + c] (i < kl)
+// -> yr[i], yi[i] (residues mod b_i of the re / imag coefficient).
+__device__ __forceinline__ void hps_scale_slot(const cp_gi *__restrict__ aq, size_t qs,
+                                               const cp_gi *__restrict__ ab, size_t bs, int c,
+                                               const cp_hps *__restrict__ H,
+                                               uint64_t (&yr)[CP_MAX_BASE],
+                                               uint64_t (&yi)[CP_MAX_BASE]) {
+  const int k = H->k, kl = H->kl;
+  cp_gi x[CP_MAX_BASE];
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++)
+    if (j < k) x[j] = aq[(size_t)j * qs + c];
+  const double two_m64 = 0x1p-64;  // __drcp_rn(2^64)
+#pragma unroll
+  for (int i = 0; i < CP_MAX_BASE; i++) {
+    if (i < kl) {
+      const uint64_t b = H->b[i].p;
+      double acc_r = 0.0, acc_i = 0.0;
+      uint64_t rr = 0, ri = 0;
+#pragma unroll
+      for (int j = 0; j < CP_MAX_BASE; j++) {
+        if (j < k) {
+          const uint64_t int_r = shoup2(x[j].re, H->wint[i][j], b);
+          const uint64_t int_i = shoup2(x[j].imag, H->wint[i][j], b);
+          uint64_t lr, hr, li, hi;
+          umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, hr);
+          umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, hi);
+          acc_r = fma((double)lr, two_m64, acc_r);  // __fma_rn (cudaengine.cu:413-414)
+          acc_i = fma((double)li, two_m64, acc_i);
+          rr = addmod_ref(rr, addmod_ref(int_r, hr, b), b);
+          ri = addmod_ref(ri, addmod_ref(int_i, hi, b), b);
+        }
+      }
+      rr = addmod_ref(rr, (uint64_t)llrint(acc_r), b);
+      ri = addmod_ref(ri, (uint64_t)llrint(acc_i), b);
+      const cp_gi y = ab[(size_t)i * bs + c];
+      yr[i] = addmod_ref(rr, shoup2(y.re, H->lam[i], b), b);
+      yi[i] = addmod_ref(ri, shoup2(y.imag, H->lam[i], b), b);
+    }
+  }
+}
+
+// B -> Q for slot c from residues xr[i], xi[i] mod b_i -> out[j*qs + c] (j < k).
+__device__ __forceinline__ void hps_b_to_q_slot(const uint64_t (&xr)[CP_MAX_BASE],
+                                                const uint64_t (&xi)[CP_MAX_BASE], cp_gi *out,
+                                                size_t qs, int c, const cp_hps *__restrict__ H) {
+  const int k = H->k, kl = H->kl;
+  uint64_t ar[CP_MAX_BASE], ai[CP_MAX_BASE];
+  double vr = 0.0, vi = 0.0;
+  // v = RN(v + RN(aux_i) / RN(b_i)) in ascending i (cudaengine.cu:699-700)
+#pragma unroll
+  for (int i = 0; i < CP_MAX_BASE; i++) {
+    if (i < kl) {
+      const uint64_t b = H->b[i].p;
+      ar[i] = shoup2(xr[i], H->invBbi[i], b);
+      ai[i] = shoup2(xi[i], H->invBbi[i], b);
+      const double bd = (double)b;
+      vr = vr + (double)ar[i] / bd;
+      vi = vi + (double)ai[i] / bd;
+    }
+  }
+  const uint64_t v_re = snap(vr), v_im = snap(vi);
+  for (int j = 0; j < k; j++) {
+    const uint64_t q = H->q[j].p;
+    uint64_t sr = 0, si = 0;
+#pragma unroll
+    for (int i = 0; i < CP_MAX_BASE; i++) {
+      if (i < kl) {
+        sr = addmod_ref(sr, shoup2(ar[i], H->Bbiqi[j][i], q), q);
+        si = addmod_ref(si, shoup2(ai[i], H->Bbiqi[j][i], q), q);
+      }
+    }
+    sr = submod_ref(sr, shoup2(v_re, H->Bqi[j], q), q);
+    si = submod_ref(si, shoup2(v_im, H->Bqi[j], q), q);
+    cp_gi o;
+    o.re = sr;
+    o.imag = si;
+    out[(size_t)j * qs + c] = o;
+  }
+}

@@ -53,6 +53,8 @@ struct cupoly_ctx {
   uint64_t *d_aux = nullptr;   // uint64 [kl][n]  (ctx->d_coefs_B in the reference)
   cp_gi *d_tmp = nullptr;      // [nres][ng]      (ctx->d_tmp_data / d_aux_coefs)
   cp_gi *d_tmp2 = nullptr;     // [nres][ng]
+  void *d_ws = nullptr;        // grow-only workspace for batched pipelines
+  size_t ws_bytes = 0;
 };
 
 // error plumbing
@@ -72,3 +74,4 @@ bool cp_base_range(const cupoly_engine *e, int base, int *off, int *cnt);
 int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int direction);
 int cp_polymul_fused_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch);
 int cp_dgt_setup(cupoly_engine *e);
+int cp_workspace(cupoly_ctx *c, size_t bytes, void **out);

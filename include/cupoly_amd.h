@@ -146,6 +146,16 @@ int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, cons
  * left in the DGT domain afterwards. */
 int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi *b, int batch);
 
+/* Batched BFV multiplication core (no single reference entry point: it is the
+ * composition poly_basis_extension_Q_to_B x4 -> poly_dr2 -> inverse DGT ->
+ * poly_complex_scaling_tDivQ x3, polynomial.cu:350-457, 951-999).  For each
+ * pair i: ct1[i] = (c0, c1), ct2[i] = (d0, d1), [batch][2][k][n/2] GIs, base Q,
+ * coefficient domain; out[i] = [3][k][n/2]: round(t/Q (c0 d0, c0 d1 + c1 d0,
+ * c1 d1)) mod Q, coefficient domain.  Uses a context-owned workspace of
+ * batch * 7 * (k+kl) * n/2 GIs (allocated on first use, grown as needed). */
+int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, const cupoly_gi *ct2,
+                         int batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -304,3 +304,25 @@ def crt(P, x_q_gi):
             acc += int(flat[j, c]) * P.invQqi[j] % qj * Mj
         out.append(acc % P.Q)
     return out
+
+
+def hom_mul(P, ct1, ct2):
+    """BFV multiplication core restated from the reference's pieces
+    (polynomial.cu:350-457, 951-999): Q->B of the four inputs, DGT in QB,
+    dr2 tensor, inverse DGT, complex scaling and B->Q.  ct1, ct2 GI
+    [2, k, Ng, 2] base Q coefficient domain -> [3, k, Ng, 2]."""
+    k = P.k
+    spec = []
+    for x in (ct1[0], ct1[1], ct2[0], ct2[1]):
+        xb = basis_ext_q_to_b(P, x)
+        spec.append(dgt(P, np.concatenate([u64(x), xb]), 2, 0))
+    c0, c1, d0, d1 = spec
+    z0 = pointwise(P, c0, d0, base=2, op=2)
+    z2 = pointwise(P, c1, d1, base=2, op=2)
+    z1 = pointwise(P, pointwise(P, c0, d1, base=2, op=2), pointwise(P, c1, d0, base=2, op=2),
+                   base=2, op=0)
+    out = []
+    for z in (z0, z1, z2):
+        zc = dgt(P, z, 2, 1)
+        out.append(basis_ext_b_to_q(P, complex_scaling(P, zc[:k], zc[k:])))
+    return np.stack(out)

@@ -213,3 +213,18 @@ def test_large_batch_properties(gpu):
     h = cp.to_host(o1)
     for i in (0, 31, 63):
         assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
+
+
+@pytest.mark.parametrize("k,n,batch", [(2, 2048, 3), (4, 16384, 2)])
+def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
+    """Config-2 shape (n = 2^14, k = 4, kl = 5, t = 65537): tensor product + HPS
+    rescale of ciphertext pairs, bit-exact against the oracle composition."""
+    e, ctx = engine(k, k + 1, n, 65537)
+    P = O.Params(k, k + 1, n, 65537)
+    ct1 = rand_res(P, P.q, (batch, 2), 90 + k)
+    ct2 = rand_res(P, P.q, (batch, 2), 91 + k)
+    out = cp.empty((batch, 3, k, P.Ng, 2))
+    cp.hom_mul_batch(ctx, out, cp.to_device(ct1), cp.to_device(ct2))
+    h = cp.to_host(out)
+    for i in range(batch):
+        assert np.array_equal(h[i], O.hom_mul(P, ct1[i], ct2[i])), i

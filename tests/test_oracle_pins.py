@@ -474,3 +474,42 @@ def test_uniform_mod_q_range_and_chi_square():
         bins = (u[r] // np.uint64(p // 16 + 1)).astype(np.int64)
         counts = np.bincount(bins, minlength=16)
         assert chi2_stat(list(counts), [1 / 16] * 16) < 37.7  # df 15, p = 0.001
+
+
+def test_hom_mul_composition_matches_exact_rescaled_tensor():
+    """HPS multiplication core against exact integers: with inputs lifted to
+    [0, Q) and z the signed negacyclic tensor product over Z, the result is
+    round(t * z / Q) mod q_j within +-1.  (z < 0 is held as z + QB; scaling
+    gives a value just below B, which the B -> Q snap maps back to t z / Q.)"""
+    k, n, t = 2, 64, 65537
+    P = O.Params(k, k + 1, n, t)
+    rng = random.Random(77)
+    polys = [[rng.randrange(P.Q) for _ in range(n)] for _ in range(4)]
+
+    def res(vals):
+        return O.fold(np.array([[v % q for v in vals] for q in P.q], dtype=np.uint64), P.Ng)
+
+    ct1 = np.stack([res(polys[0]), res(polys[1])])
+    ct2 = np.stack([res(polys[2]), res(polys[3])])
+    got = O.hom_mul(P, ct1, ct2)
+
+    def negamul(a, b):
+        c = [0] * n
+        for i in range(n):
+            for j in range(n):
+                if i + j < n:
+                    c[i + j] += a[i] * b[j]
+                else:
+                    c[i + j - n] -= a[i] * b[j]
+        return c
+
+    c0, c1, d0, d1 = polys
+    z = [negamul(c0, d0), [x + y for x, y in zip(negamul(c0, d1), negamul(c1, d0))],
+         negamul(c1, d1)]
+    for m in range(3):
+        flat = O.unfold(got[m])
+        for c in range(n):
+            y = (2 * t * z[m][c] + P.Q) // (2 * P.Q)
+            for j, q in enumerate(P.q):
+                d = (int(flat[j, c]) - y) % q
+                assert d in (0, 1, q - 1), (m, c, j)
