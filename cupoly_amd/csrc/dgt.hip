@@ -28,23 +28,32 @@
 #include "internal.hpp"
 
 #define EPT 8       // Gaussian slots per thread
+#ifndef CP_STAGE_BARRIER
+#define CP_STAGE_BARRIER 1
+#endif
 #define LOG_EPT 3
 
 // -------------------------------------------------------------------------
 // Index maps: local slot index (inside the workgroup's LDS array) -> global
 // slot index inside one residue polynomial.
 // -------------------------------------------------------------------------
+// kAligned: global index == local index modulo 8 (true for whole polynomials
+// and 128-slot blocks), so in a last pass of unit stride 1 the twiddles of the
+// final three stages are known at compile time.
 struct MapIdentity {  // whole polynomial, local == global
+  static constexpr bool kAligned = true;
   __device__ __forceinline__ int operator()(int l) const { return l; }
 };
 template <int LOGC, int LOGNS>  // 2^LOGC columns of a polynomial, column stride 2^LOGNS
 struct MapColumns {
+  static constexpr bool kAligned = false;
   int col0;
   __device__ __forceinline__ int operator()(int l) const {
     return col0 + (l & ((1 << LOGC) - 1)) + ((l >> LOGC) << LOGNS);
   }
 };
-struct MapBlock {  // contiguous block starting at base
+struct MapBlock {  // contiguous block starting at base (a multiple of 8)
+  static constexpr bool kAligned = true;
   int base;
   __device__ __forceinline__ int operator()(int l) const { return base + l; }
 };
@@ -58,6 +67,9 @@ __device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, int e, ui
 
 // One forward pass over local stages [LS0, LS0+R) of a 2^LOGM local array.
 // Global stage = local stage + SOFF; LOGN = log2 of the global transform size.
+// Butterflies whose twiddle is 1 are done without a multiply: always in the
+// last global stage, and, when the unit stride is 1 on an aligned map, every
+// butterfly whose index within its group is 0.
 template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
 __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
                                          uint64_t p, const Map &map) {
@@ -70,6 +82,7 @@ __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t 
     for (int h = 0; h < R; h++) {
       const int half = U >> (h + 1);
       const int s = LS0 + h + SOFF;
+      const int m = 1 << (LOGN - 1 - s);
 #pragma unroll
       for (int q = 0; q < U; q++) {
         if (q & half) continue;
@@ -79,18 +92,20 @@ __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t 
         const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
         A.re = cp_red2(sr, p);
         A.imag = cp_red2(si, p);
-        if (s == LOGN - 1) {  // twiddle 1
+        const bool static_j = Map::kAligned && ST == 1 && m <= U;
+        const int jc = q & (m - 1);
+        if (s == LOGN - 1 || (static_j && jc == 0)) {  // twiddle 1
           B.re = cp_red2(dr, p);
           B.imag = cp_red2(di, p);
         } else {
-          const int g = map(base + q * ST);
-          const int e = (g & ((1 << (LOGN - 1 - s)) - 1)) << s;
+          const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
           uint64_t w, wp;
           ld_tw(tw, e, w, wp);
           B.re = cp_shoup(dr, w, wp, p);
           B.imag = cp_shoup(di, w, wp, p);
         }
       }
+      if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -108,18 +123,20 @@ __device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, const uint64_t 
     for (int h = R - 1; h >= 0; h--) {
       const int half = U >> (h + 1);
       const int s = LS0 + h + SOFF;
+      const int m = 1 << (LOGN - 1 - s);
 #pragma unroll
       for (int q = 0; q < U; q++) {
         if (q & half) continue;
         cp_gi &A = x[ju * U + q];
         cp_gi &B = x[ju * U + q + half];
         uint64_t tr, ti;
-        if (s == LOGN - 1) {
+        const bool static_j = Map::kAligned && ST == 1 && m <= U;
+        const int jc = q & (m - 1);
+        if (s == LOGN - 1 || (static_j && jc == 0)) {
           tr = B.re;
           ti = B.imag;
         } else {
-          const int g = map(base + q * ST);
-          const int e = (g & ((1 << (LOGN - 1 - s)) - 1)) << s;
+          const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
           uint64_t w, wp;
           ld_tw(tw, e, w, wp);
           tr = cp_shoup(B.re, w, wp, p);
@@ -130,6 +147,7 @@ __device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, const uint64_t 
         A.re = cp_red2(A.re + tr, p);
         A.imag = cp_red2(A.imag + ti, p);
       }
+      if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -199,6 +217,8 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, const uint64
       B.re = cp_shoup(dr, w, wp, p);
       B.imag = cp_shoup(di, w, wp, p);
     }
+    // one stage's twiddles in flight at a time (bounds VGPR use)
+    if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -227,64 +247,88 @@ __device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, const uint64
       A.re = cp_red2(A.re + tr, p);
       A.imag = cp_red2(A.imag + ti, p);
     }
+    if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// Pass plan over local stages [LSB, LSE): NP passes; all but the last are
-// full radix-8 passes run in a loop that is not unrolled (so the compiler
-// cannot hoist twiddle loads across passes); the last pass has RL stages.
+// Pass plan over local stages [LSB, LSE): an optional partial pass of
+// R0 = NS mod 3 stages first, then NF full radix-8 passes.  The full passes
+// but the last run in a loop that is not unrolled (so the compiler cannot
+// hoist twiddle loads across passes and spill); the last pass is compiled
+// with its stages known, which lets its trivial twiddles drop out.
 template <int LSB, int LSE>
 struct PassPlan {
   static constexpr int NS = LSE - LSB;
-  static constexpr int NP = (NS + LOG_EPT - 1) / LOG_EPT;
-  static constexpr int LSL = LSB + LOG_EPT * (NP - 1);  // first stage of the last pass
-  static constexpr int RL = LSE - LSL;                  // stages in the last pass
+  static constexpr int R0 = NS % LOG_EPT;                  // partial first pass (0: none)
+  static constexpr int NF = NS / LOG_EPT;                  // full passes
+  static constexpr int LSF = LSB + R0;                     // first stage of the full passes
+  static constexpr int LSL = NF ? LSE - LOG_EPT : LSB;     // first stage of the last pass
+  static constexpr int RL = NF ? LOG_EPT : R0;             // stages in the last pass
+  static constexpr int LS_FIRST = LSB;                     // first pass
+  static constexpr int R_FIRST = R0 ? R0 : LOG_EPT;
 };
 
-// Forward passes over local stages [LSB, LSE), starting from registers in the
-// layout of the first pass (natural layout when LSB == 0), ending in the
-// layout of the last pass.
+// Register layout of the first pass (used for direct global I/O).
+template <int LOGM, int LSB, int LSE>
+__device__ __forceinline__ int first_index(int t, int i) {
+  using PP = PassPlan<LSB, LSE>;
+  return pass_index<LOGM, PP::LS_FIRST, PP::R_FIRST>(t, i);
+}
+
+// Forward passes over local stages [LSB, LSE), registers starting in the
+// first-pass layout and ending in the last-pass layout.
 template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
 __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            const uint64_t *__restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
+  if constexpr (PP::R0 > 0) {
+    dif_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
+    if constexpr (PP::NF == 0) return;
+    __syncthreads();
+    lds_store<LOGM, LSB, PP::R0>(lds, x, t);
+    __syncthreads();
+    lds_load_rt<LOGM>(lds, x, t, PP::LSF);
+  }
 #pragma nounroll
-  for (int ps = 0; ps < PP::NP - 1; ps++) {
-    const int ls0 = LSB + LOG_EPT * ps;
+  for (int i = 0; i < PP::NF - 1; i++) {
+    const int ls0 = PP::LSF + LOG_EPT * i;
     dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
     __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0);
     __syncthreads();
-    if (ps + 1 < PP::NP - 1)
-      lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
-    else
-      lds_load<LOGM, PP::LSL, PP::RL>(lds, x, t);
+    lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
   }
-  dif_pass<LOGN, LOGM, PP::LSL, PP::RL, SOFF>(x, t, tw, p, map);
+  if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 }
 
 // Inverse passes over local stages [LSB, LSE) in reverse order; registers
-// start in the layout of the last pass and end in the layout of the first.
+// start in the last-pass layout and end in the first-pass layout.
 template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
 __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            const uint64_t *__restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
-  dit_pass<LOGN, LOGM, PP::LSL, PP::RL, SOFF>(x, t, tw, p, map);
+  if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 #pragma nounroll
-  for (int ps = PP::NP - 2; ps >= 0; ps--) {
-    const int ls0 = LSB + LOG_EPT * ps;
+  for (int i = PP::NF - 2; i >= 0; i--) {
+    const int ls0 = PP::LSF + LOG_EPT * i;
     __syncthreads();
-    if (ps + 1 == PP::NP - 1)
-      lds_store<LOGM, PP::LSL, PP::RL>(lds, x, t);
-    else
-      lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
+    lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
     __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0);
     dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
+  }
+  if constexpr (PP::R0 > 0) {
+    if constexpr (PP::NF > 0) {
+      __syncthreads();
+      lds_store_rt<LOGM>(lds, x, t, PP::LSF);
+      __syncthreads();
+      lds_load<LOGM, LSB, PP::R0>(lds, x, t);
+    }
+    dit_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
   }
 }
 
@@ -295,7 +339,7 @@ struct LastPass {
   static constexpr int RL = PassPlan<LSB, LSE>::RL;
 };
 
-// natural layout index of register i: t + i*T (== first-pass layout)
+// natural layout index of register i: t + i*T (LDS-staged global I/O)
 template <int LOGM>
 __device__ __forceinline__ int nat_index(int t, int i) {
   return t + i * ((1 << LOGM) / EPT);
@@ -319,12 +363,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
                  const uint64_t *__restrict__ twist_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
-  const int row = threadIdx.x / T, t = threadIdx.x % T;
+  // with one row per workgroup, keep the row (hence the residue and every
+  // table pointer) wave-uniform so it lives in SGPRs
+  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
+  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
   cp_gi *lds = lds_all + row * N;
   int unit = blockIdx.x * ROWS + row;
   const bool active = unit < nunits;
   if (!active) unit = nunits - 1;
-  const int rid = roff + unit % nres;
+  int rid = roff + unit % nres;
+  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);  // uniform: keep in SGPRs
   const uint64_t p = primes[rid].p;
   const uint64_t *tw = tw_all + (size_t)rid * N;          // N/2 pairs
   const uint64_t *tt = twist_all + (size_t)rid * N * 6;
@@ -334,7 +382,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   if constexpr (FWD) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-      const int g = nat_index<LOGN>(t, i);
+      const int g = first_index<LOGN, 0, LOGN>(t, i);
       x[i] = poly[g];
       twist(x[i], tt, g, p);
     }
@@ -357,7 +405,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
     if (active) {
 #pragma unroll
       for (int i = 0; i < EPT; i++) {
-        const int g = nat_index<LOGN>(t, i);
+        const int g = first_index<LOGN, 0, LOGN>(t, i);
         twist(x[i], tt, g, p);
         poly[g] = x[i];
       }
@@ -404,7 +452,7 @@ __global__ void __launch_bounds__(1024)
   if constexpr (FWD) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-      const int g = map(nat_index<LOGM>(t, i));
+      const int g = map(first_index<LOGM, 0, SP::S>(t, i));
       x[i] = poly[g];
       twist(x[i], tt, g, p);
     }
@@ -424,7 +472,7 @@ __global__ void __launch_bounds__(1024)
     dit_passes<LOGN, LOGM, 0, SP::S, 0>(x, lds, t, tw, p, map);
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-      const int g = map(nat_index<LOGM>(t, i));
+      const int g = map(first_index<LOGM, 0, SP::S>(t, i));
       twist(x[i], tt, g, p);
       poly[g] = x[i];
     }
@@ -454,7 +502,7 @@ __global__ void __launch_bounds__(1024)
   using LP = LastPass<LOGB, 0, LOGB>;
   if constexpr (FWD) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) x[i] = poly[map(nat_index<LOGB>(t, i))];
+    for (int i = 0; i < EPT; i++) x[i] = poly[map(first_index<LOGB, 0, LOGB>(t, i))];
     dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
     __syncthreads();
     lds_store<LOGB, LP::LS0, LP::RL>(lds, x, t);
@@ -468,7 +516,7 @@ __global__ void __launch_bounds__(1024)
     lds_load<LOGB, LP::LS0, LP::RL>(lds, x, t);
     dit_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
 #pragma unroll
-    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGB>(t, i))] = x[i];
+    for (int i = 0; i < EPT; i++) poly[map(first_index<LOGB, 0, LOGB>(t, i))] = x[i];
   }
   (void)nunits;
 }
@@ -486,12 +534,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
                     const uint64_t *__restrict__ tsi_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
-  const int row = threadIdx.x / T, t = threadIdx.x % T;
+  // with one row per workgroup, keep the row (hence the residue and every
+  // table pointer) wave-uniform so it lives in SGPRs
+  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
+  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
   cp_gi *lds = lds_all + row * N;
   int unit = blockIdx.x * ROWS + row;
   const bool active = unit < nunits;
   if (!active) unit = nunits - 1;
-  const int rid = unit % nres;
+  int rid = unit % nres;
+  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);  // uniform: keep in SGPRs
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
   const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
@@ -499,27 +551,43 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   const size_t off = (size_t)unit * N;
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
+  // a: load, twist, transform; b is loaded only afterwards so that a single
+  // operand is live during each forward transform.  The scheduling barriers
+  // keep the twist-table loads of one slot from being hoisted above the
+  // previous slot's products (which would spill at 128 VGPRs).
+#pragma unroll
+  for (int i = 0; i < EPT; i++) xa[i] = a[off + first_index<LOGN, 0, LOGN>(t, i)];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
-    const int g = nat_index<LOGN>(t, i);
-    xa[i] = a[off + g];
-    xb[i] = b[off + g];
+    twist(xa[i], tsf, first_index<LOGN, 0, LOGN>(t, i), p);
+    __builtin_amdgcn_sched_barrier(0);
   }
-#pragma unroll
-  for (int i = 0; i < EPT; i++) twist(xa[i], tsf, nat_index<LOGN>(t, i), p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
+  // an opaque copy of the thread index: the address arithmetic of the later
+  // loads and stores cannot be hoisted above this point (and held in VGPRs
+  // across the transform).
+  int tb = t;
+  asm volatile("" : "+v"(tb));
 #pragma unroll
-  for (int i = 0; i < EPT; i++) twist(xb[i], tsf, nat_index<LOGN>(t, i), p);
+  for (int i = 0; i < EPT; i++) xb[i] = b[off + first_index<LOGN, 0, LOGN>(tb, i)];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    twist(xb[i], tsf, first_index<LOGN, 0, LOGN>(tb, i), p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
   for (int i = 0; i < EPT; i++) xa[i] = cp_gimul(xa[i], xb[i], pr);
   dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twi, p, id);
+  int to = t;
+  asm volatile("" : "+v"(to));
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-      const int g = nat_index<LOGN>(t, i);
+      const int g = first_index<LOGN, 0, LOGN>(to, i);
       twist(xa[i], tsi, g, p);
       out[off + g] = xa[i];
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -548,8 +616,8 @@ __global__ void __launch_bounds__(1024)
   cp_gi xa[EPT], xb[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
-    xa[i] = pa[map(nat_index<LOGB>(t, i))];
-    xb[i] = pb[map(nat_index<LOGB>(t, i))];
+    xa[i] = pa[map(first_index<LOGB, 0, LOGB>(t, i))];
+    xb[i] = pb[map(first_index<LOGB, 0, LOGB>(t, i))];
   }
   dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(xa, lds, t, twf, p, map);
   dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(xb, lds, t, twf, p, map);
@@ -557,7 +625,7 @@ __global__ void __launch_bounds__(1024)
   for (int i = 0; i < EPT; i++) xa[i] = cp_gimul(xa[i], xb[i], pr);
   dit_passes<LOGN, LOGB, 0, LOGB, SP::S>(xa, lds, t, twi, p, map);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) pa[map(nat_index<LOGB>(t, i))] = xa[i];
+  for (int i = 0; i < EPT; i++) pa[map(first_index<LOGB, 0, LOGB>(t, i))] = xa[i];
 }
 
 // -------------------------------------------------------------------------

@@ -10,8 +10,9 @@
 //   * products of two variables use Moller-Granlund division by the
 //     normalised divisor 2p with a precomputed inverse ("Improved division by
 //     invariant integers", IEEE TC 2011, Alg. 4), then one correction.
-// Both need p in (2^62, 2^63), which every prime of the 63-bit bucket meets
-// (the engine checks it).  On gfx950 the 128-bit products lower to
+// Both need p in (2^62, 2^63); the Shoup remainder also uses p = 2^63 - delta
+// with delta < 2^32.  Every prime of the 63-bit bucket meets both (the engine
+// checks).  On gfx950 the 128-bit products lower to
 // v_mad_u64_u32 / v_mul_hi_u32 chains on the integer pipes.
 #pragma once
 #include <stdint.h>
@@ -46,13 +47,26 @@ CP_HD uint64_t cp_sub(uint64_t a, uint64_t b, uint64_t p) {
   return a >= b ? a - b : a + (p - b);
 }
 
-// Reduce from [0, 2p) to [0, p).
-CP_HD uint64_t cp_red2(uint64_t a, uint64_t p) { return a >= p ? a - p : a; }
+// Reduce from [0, 2p) to [0, p): the borrow of a - p selects (no separate compare).
+CP_HD uint64_t cp_red2(uint64_t a, uint64_t p) {
+  uint64_t t;
+  return __builtin_sub_overflow(a, p, &t) ? a : t;
+}
 
 // Shoup: a * w mod p in [0, 2p) for any a < 2^64, w < p, wp = floor(w 2^64 / p).
+// The remainder a w - q p is formed mod 2^64 as a w + q (2^64 - p); with
+// p = 2^63 - delta, delta < 2^32 (every prime of the 63-bit bucket), that is
+// a w + q delta + (q mod 2) 2^63: two 32x32+64 multiply-adds for the low word
+// and three 32-bit products for the high word, and no 64-bit subtraction.
 CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
-  uint64_t q = cp_mulhi(a, wp);
-  return a * w - q * p;
+  const uint64_t q = cp_mulhi(a, wp);
+  const uint32_t delta = (uint32_t)((1ULL << 63) - p);
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+  const uint32_t q0 = (uint32_t)q, q1 = (uint32_t)(q >> 32);
+  const uint64_t x = (uint64_t)a0 * w0 + (uint64_t)q0 * delta;
+  const uint32_t hi = (uint32_t)(x >> 32) + a0 * w1 + a1 * w0 + q1 * delta + (q0 << 31);
+  return ((uint64_t)hi << 32) | (uint32_t)x;
 }
 
 CP_HD uint64_t cp_shoup(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
@@ -70,7 +84,7 @@ CP_HD uint64_t cp_rem2(uint64_t u1, uint64_t u0, const cp_prime& pr) {
   return r;
 }
 
-// a * b mod p for a, b < 2^63 (so a*b < 2^126 and the high word < 2p).
+// a * b mod p for a < 2^64, b < 2^63 (so the high word of a*b is below 2p).
 CP_HD uint64_t cp_mulmod(uint64_t a, uint64_t b, const cp_prime& pr) {
   cp_u128 x = (cp_u128)a * b;
   uint64_t r = cp_rem2((uint64_t)(x >> 64), (uint64_t)x, pr);

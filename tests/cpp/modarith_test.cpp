@@ -1,0 +1,54 @@
+// CPU check of the modular primitives in cupoly_amd/csrc/modarith.hpp (the
+// same inline functions the gfx950 kernels use) against unsigned __int128 %.
+// usage: modarith_test <primes...>; prints "modarith ok" or the first failure.
+#define CP_HD static inline
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../cupoly_amd/csrc/modarith.hpp"
+
+static uint64_t rng_state = 0x9E3779B97F4A7C15ULL;
+static uint64_t next64() {  // splitmix64
+  uint64_t z = (rng_state += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int main(int argc, char **argv) {
+  long checks = 0;
+  for (int i = 1; i < argc; i++) {
+    const uint64_t p = strtoull(argv[i], nullptr, 10);
+    const cp_prime pr = cp_make_prime(p);
+    std::vector<uint64_t> edge = {0, 1, 2, p - 1, p - 2, p, p + 1, 2 * p - 1, (1ULL << 63) - 1,
+                                  1ULL << 63, ~0ULL, ~0ULL - 1};
+    for (int it = 0; it < 400000; it++) {
+      uint64_t a = it < 144 ? edge[it % 12] : next64();
+      uint64_t w = it < 144 ? edge[(it / 12) % 12] % p : next64() % p;
+      const uint64_t want = (uint64_t)(((cp_u128)a * w) % p);
+      const uint64_t wp = cp_shoup_factor(w, p);
+      const uint64_t lazy = cp_shoup_lazy(a, w, wp, p);
+      if (lazy >= 2 * p || lazy % p != want || cp_shoup(a, w, wp, p) != want) {
+        std::printf("shoup FAIL p=%llu a=%llu w=%llu\n", (unsigned long long)p,
+                    (unsigned long long)a, (unsigned long long)w);
+        return 1;
+      }
+      const uint64_t b = it < 144 ? w : next64() >> 1;  // b < 2^63
+      const uint64_t want2 = (uint64_t)(((cp_u128)a * b) % p);
+      if (cp_mulmod(a, b, pr) != want2) {
+        std::printf("mulmod FAIL p=%llu a=%llu b=%llu\n", (unsigned long long)p,
+                    (unsigned long long)a, (unsigned long long)b);
+        return 1;
+      }
+      const uint64_t r = next64() % (2 * p);
+      if (cp_red2(r, p) != r % p) {
+        std::printf("red2 FAIL\n");
+        return 1;
+      }
+      checks += 3;
+    }
+  }
+  std::printf("modarith ok (%ld checks)\n", checks);
+  return 0;
+}
