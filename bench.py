@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--hm-batch", type=int, default=128,
                     help="ciphertext pairs per batch for the BFV multiplication core")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=20000.0,
+                    help="offered load of the serving run, requests/s (Poisson)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -141,6 +144,24 @@ def main():
         del ct1, ct2, hout
         print("bench: hom-mul done %.0f hom-mults/s" % (HB / hm_s), file=sys.stderr, flush=True)
 
+    # serving: Poisson arrivals, 70% (n=2^13, k=3) / 30% (n=2^15, k=8) BFV
+    # multiplications, continuous batching, HBM-footprint admission
+    serve = None
+    if args.serve_requests > 0:
+        from cupoly_amd import serving
+        shapes = [serving.Shape(8192, 3), serving.Shape(32768, 8)]
+        reqs = serving.make_requests(shapes, [0.7, 0.3], args.serve_requests, args.serve_rate,
+                                     seed=2024)
+        srv = serving.Server(shapes, hbm_budget=32 << 30,
+                             max_batch={shapes[0]: 64, shapes[1]: 16})
+        serve = srv.run(reqs)
+        serve.update({"offered_req_per_s": args.serve_rate,
+                      "mix": "70% n=8192 k=3 / 30% n=32768 k=8, t=65537, tensor + HPS rescale",
+                      "hbm_budget_bytes": 32 << 30})
+        del reqs, srv, shapes
+        torch.cuda.synchronize()
+        print("bench: serving done p99 %.3f ms" % serve["p99_ms"], file=sys.stderr, flush=True)
+
     # HBM bytes per launch from the committed rocprofv3 PMC summary of this
     # same configuration (FETCH_SIZE / WRITE_SIZE scaled by a copy of known size)
     traffic = None
@@ -192,6 +213,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "hom_mul": hm,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "cupoly_polymul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_polymul_plain": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
 }
 
 _lib = None
@@ -332,6 +333,23 @@ def hom_mul_batch(ctx, out, ct1, ct2):
         if tuple(t.shape) != (batch, m, e.k, e.N, 2):
             raise ValueError("cupoly_amd: hom_mul_batch operand shape %s" % (tuple(t.shape),))
     _check(lib().cupoly_hom_mul_batch(ctx.handle, _ptr(out), _ptr(ct1), _ptr(ct2), batch))
+
+
+def hom_mul_gather(ctx, outs, ct1s, ct2s):
+    """hom_mul_batch over lists of per-request tensors (out [3, k, N, 2],
+    ct [2, k, N, 2]), which need not be contiguous with each other."""
+    e = ctx.engine
+    b = len(outs)
+    if not (len(ct1s) == len(ct2s) == b):
+        raise ValueError("cupoly_amd: hom_mul_gather list lengths differ")
+    for t, m in [(x, 3) for x in outs] + [(x, 2) for x in ct1s] + [(x, 2) for x in ct2s]:
+        if tuple(t.shape) != (m, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: hom_mul_gather operand shape %s" % (tuple(t.shape),))
+    arr = ctypes.c_void_p * max(b, 1)
+    po = arr(*[t.data_ptr() for t in outs])
+    p1 = arr(*[_ptr(t).value for t in ct1s])
+    p2 = arr(*[_ptr(t).value for t in ct2s])
+    _check(lib().cupoly_hom_mul_gather(ctx.handle, po, p1, p2, b))
 
 
 # ---- host <-> device helpers ------------------------------------------------

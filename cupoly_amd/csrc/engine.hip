@@ -351,6 +351,10 @@ extern "C" int cupoly_ctx_destroy(cupoly_ctx *c) {
   if (c->d_tmp) (void)hipFree(c->d_tmp);
   if (c->d_tmp2) (void)hipFree(c->d_tmp2);
   if (c->d_ws) (void)hipFree(c->d_ws);
+  if (c->d_ptrs) (void)hipFree(c->d_ptrs);
+  if (c->h_ring) (void)hipHostFree(c->h_ring);
+  for (int i = 0; i < cupoly_ctx::kRing; i++)
+    if (c->ring_ev[i]) (void)hipEventDestroy(c->ring_ev[i]);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUPOLY_OK;

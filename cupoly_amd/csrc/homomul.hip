@@ -20,15 +20,24 @@
 // Scratch lives in a grow-only workspace owned by the context.
 #include "hps_dev.hpp"
 
+// Where ciphertext i lives: base + i * stride, or table[i] (gathered
+// requests of a serving batch).
+struct cp_cts {
+  cp_gi *base;
+  size_t stride;
+  cp_gi *const *table;
+  __device__ __forceinline__ cp_gi *at(int i) const { return table ? table[i] : base + i * stride; }
+};
+
 // blockIdx.y = pair * 4 + which (0: c0, 1: c1, 2: d0, 3: d1)
 __global__ void __launch_bounds__(256)
-    k_ext_q_to_qb(cp_gi *__restrict__ X, const cp_gi *__restrict__ ct1,
-                  const cp_gi *__restrict__ ct2, int ng, const cp_hps *__restrict__ H) {
+    k_ext_q_to_qb(cp_gi *__restrict__ X, cp_cts ct1, cp_cts ct2, int ng,
+                  const cp_hps *__restrict__ H) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int k = H->k, kl = H->kl;
   const int pair = blockIdx.y >> 2, which = blockIdx.y & 3;
-  const cp_gi *in = (which < 2 ? ct1 : ct2) + ((size_t)pair * 2 + (which & 1)) * k * ng;
+  const cp_gi *in = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)(which & 1) * k * ng;
   cp_gi *out = X + (size_t)blockIdx.y * (k + kl) * ng;
   for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
   hps_q_to_b_slot(in, ng, out + (size_t)k * ng, ng, c, H);
@@ -60,15 +69,15 @@ __global__ void __launch_bounds__(256)
 
 // blockIdx.y = output polynomial index (pair * 3 + m)
 __global__ void __launch_bounds__(256)
-    k_scale_to_q(cp_gi *__restrict__ out, const cp_gi *__restrict__ Y, int ng,
-                 const cp_hps *__restrict__ H) {
+    k_scale_to_q(cp_cts out, const cp_gi *__restrict__ Y, int ng, const cp_hps *__restrict__ H) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int k = H->k, kl = H->kl;
+  const int pair = blockIdx.y / 3, m = blockIdx.y % 3;
   const cp_gi *y = Y + (size_t)blockIdx.y * (k + kl) * ng;
   uint64_t yr[CP_MAX_BASE], yi[CP_MAX_BASE];
   hps_scale_slot(y, ng, y + (size_t)k * ng, ng, c, H, yr, yi);
-  hps_b_to_q_slot(yr, yi, out + (size_t)blockIdx.y * k * ng, ng, c, H);
+  hps_b_to_q_slot(yr, yi, out.at(pair) + (size_t)m * k * ng, ng, c, H);
 }
 
 int cp_workspace(cupoly_ctx *c, size_t bytes, void **out) {
@@ -86,22 +95,18 @@ int cp_workspace(cupoly_ctx *c, size_t bytes, void **out) {
   return CUPOLY_OK;
 }
 
-extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
-                                    const cupoly_gi *ct2, int batch) {
-  if (!c || !out || !ct1 || !ct2 || batch < 0)
-    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch: bad argument");
-  if (batch == 0) return CUPOLY_OK;
+static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
   cupoly_engine *e = c->e;
   const int k = e->k, kl = e->kl, nres = k + kl, ng = e->ng;
-  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch: batch > 16383");
+  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   const size_t xs = (size_t)batch * 4 * nres * ng, ys = (size_t)batch * 3 * nres * ng;
   void *ws;
   int rc = cp_workspace(c, (xs + ys) * sizeof(cp_gi), &ws);
   if (rc) return rc;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
-  hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X,
-                     (const cp_gi *)ct1, (const cp_gi *)ct2, ng, e->d_hps);
+  hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1, ct2,
+                     ng, e->d_hps);
   CP_LAUNCH_CHECK();
   if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
   const long total = (long)batch * nres * ng;
@@ -111,8 +116,68 @@ extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
                      e->d_primes);
   CP_LAUNCH_CHECK();
   if ((rc = cp_dgt_launch(c, Y, 0, nres, batch * 3, CUPOLY_INVERSE))) return rc;
-  hipLaunchKernelGGL(k_scale_to_q, dim3(sg.x, batch * 3), dim3(256), 0, c->stream,
-                     (cp_gi *)out, Y, ng, e->d_hps);
+  hipLaunchKernelGGL(k_scale_to_q, dim3(sg.x, batch * 3), dim3(256), 0, c->stream, out, Y, ng,
+                     e->d_hps);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
+}
+
+extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
+                                    const cupoly_gi *ct2, int batch) {
+  if (!c || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch: bad argument");
+  if (batch == 0) return CUPOLY_OK;
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_cts o{(cp_gi *)out, 3 * k * ng, nullptr}, a{(cp_gi *)ct1, 2 * k * ng, nullptr},
+      b{(cp_gi *)ct2, 2 * k * ng, nullptr};
+  return hom_mul_run(c, o, a, b, batch);
+}
+
+extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
+                                     const cupoly_gi *const *ct1, const cupoly_gi *const *ct2,
+                                     int batch) {
+  if (!c || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
+  if (batch == 0) return CUPOLY_OK;
+  for (int i = 0; i < batch; i++)
+    if (!out[i] || !ct1[i] || !ct2[i]) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: null entry");
+  // pointer tables to the device on the context's stream, from a pinned
+  // staging slot that is not rewritten until its previous copy has finished
+  const size_t need = (size_t)batch * 3 * sizeof(void *);
+  if (c->ptr_bytes < need) {
+    if (c->d_ptrs) {
+      CP_HIP(hipStreamSynchronize(c->stream));
+      CP_HIP(hipFree(c->d_ptrs));
+      c->d_ptrs = nullptr;
+      c->ptr_bytes = 0;
+    }
+    CP_HIP(hipMalloc((void **)&c->d_ptrs, need));
+    c->ptr_bytes = need;
+  }
+  if (c->ring_slot_bytes < need) {
+    if (c->h_ring) {
+      CP_HIP(hipStreamSynchronize(c->stream));
+      CP_HIP(hipHostFree(c->h_ring));
+      c->h_ring = nullptr;
+    }
+    CP_HIP(hipHostMalloc(&c->h_ring, need * cupoly_ctx::kRing, hipHostMallocDefault));
+    c->ring_slot_bytes = need;
+    for (int i = 0; i < cupoly_ctx::kRing; i++) c->ring_used[i] = false;
+  }
+  const int slot = c->ring_next;
+  c->ring_next = (slot + 1) % cupoly_ctx::kRing;
+  if (!c->ring_ev[slot]) CP_HIP(hipEventCreateWithFlags(&c->ring_ev[slot], hipEventDisableTiming));
+  if (c->ring_used[slot]) CP_HIP(hipEventSynchronize(c->ring_ev[slot]));
+  const void **h = (const void **)((char *)c->h_ring + (size_t)slot * c->ring_slot_bytes);
+  for (int i = 0; i < batch; i++) {
+    h[i] = out[i];
+    h[batch + i] = ct1[i];
+    h[2 * batch + i] = ct2[i];
+  }
+  CP_HIP(hipMemcpyAsync(c->d_ptrs, h, need, hipMemcpyHostToDevice, c->stream));
+  CP_HIP(hipEventRecord(c->ring_ev[slot], c->stream));
+  c->ring_used[slot] = true;
+  cp_gi **d = (cp_gi **)c->d_ptrs;
+  cp_cts o{nullptr, 0, d}, a{nullptr, 0, d + batch}, b{nullptr, 0, d + 2 * batch};
+  return hom_mul_run(c, o, a, b, batch);
 }

@@ -55,6 +55,16 @@ struct cupoly_ctx {
   cp_gi *d_tmp2 = nullptr;     // [nres][ng]
   void *d_ws = nullptr;        // grow-only workspace for batched pipelines
   size_t ws_bytes = 0;
+  void *d_ptrs = nullptr;      // gathered-request pointer tables
+  size_t ptr_bytes = 0;
+  // pinned staging ring for the pointer tables: slot i is reused only after
+  // the copy that read it has completed (ring_ev[i])
+  static constexpr int kRing = 8;
+  void *h_ring = nullptr;
+  size_t ring_slot_bytes = 0;
+  hipEvent_t ring_ev[kRing] = {};
+  bool ring_used[kRing] = {};
+  int ring_next = 0;
 };
 
 // error plumbing

@@ -1,0 +1,148 @@
+"""Continuous-batching request loop for BFV multiplications on one MI355X.
+
+Requests (a pair of ciphertexts of one ring shape) arrive over time.  The loop
+  * keeps one CUDAEngine per ring shape resident (twiddle, twist and HPS
+    tables stay in HBM for the whole run) and one context + stream per shape,
+    so batches of different shapes overlap on the device;
+  * admits arrived requests while their HBM footprint (output buffers plus
+    their share of the pipeline workspace) fits the budget; the rest wait;
+  * whenever a shape has fewer than `max_inflight` batches on the device, it
+    launches every admitted request of that shape (up to `max_batch`) as one
+    gathered batch (cupoly_hom_mul_gather), so requests that arrive while
+    others are on the GPU join the next batch instead of waiting for a drain;
+  * timestamps completions with device events, and reports throughput and
+    p50 / p99 latency from arrival to result-ready.
+
+The reference has no serving layer (it is a single-engine library); this is
+the caller the batched kernels are built for.
+"""
+import collections
+import time
+
+import numpy as np
+
+import cupoly_amd as cp
+
+
+class Shape:
+    def __init__(self, n, k, kl=None, t=65537):
+        import torch
+        self.n, self.k, self.kl, self.t = n, k, kl or k + 1, t
+        self.engine = cp.CUDAEngine(self.k, self.kl, n, t)
+        self.stream = torch.cuda.Stream()  # default priority
+        self.ctx = cp.Context(self.engine, stream=self.stream)
+        ng = n // 2
+        self.ct_shape = (2, self.k, ng, 2)
+        self.out_shape = (3, self.k, ng, 2)
+        # output + pipeline workspace (7 (k + kl) polys of n/2 slots, 16 B)
+        self.footprint = 3 * self.k * ng * 16 + 7 * (self.k + self.kl) * ng * 16
+
+    def __repr__(self):
+        return "n=%d,k=%d" % (self.n, self.k)
+
+
+class Request:
+    __slots__ = ("rid", "shape", "arrival", "ct1", "ct2", "out", "done_ms", "batch")
+
+    def __init__(self, rid, shape, arrival, ct1, ct2, out):
+        self.rid, self.shape, self.arrival = rid, shape, arrival
+        self.ct1, self.ct2, self.out = ct1, ct2, out
+        self.done_ms = None
+        self.batch = None
+
+
+def make_requests(shapes, probs, n_req, rate, seed, device="cuda"):
+    """Poisson arrivals at `rate` req/s; shape i with probability probs[i];
+    inputs uniform mod q_j from the Philox sampler (stream = request id)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    arrivals -= arrivals[0]
+    kinds = rng.choice(len(shapes), size=n_req, p=probs)
+    gens = {id(sh): cp.Context(sh.engine) for sh in shapes}
+    reqs = []
+    for i in range(n_req):
+        sh = shapes[kinds[i]]
+        gen = gens[id(sh)]
+        ct1 = cp.empty(sh.ct_shape, device)
+        ct2 = cp.empty(sh.ct_shape, device)
+        for j in range(2):
+            cp.Sampler.sample(gen, ct1[j], cp.UNIFORM, seed, 4 * i + j, coeff_domain=True)
+            cp.Sampler.sample(gen, ct2[j], cp.UNIFORM, seed, 4 * i + 2 + j, coeff_domain=True)
+        reqs.append(Request(i, sh, float(arrivals[i]), ct1, ct2, cp.empty(sh.out_shape, device)))
+    torch.cuda.synchronize()
+    for g in gens.values():
+        g.destroy()
+    return reqs
+
+
+class Server:
+    def __init__(self, shapes, hbm_budget, max_batch=None, max_inflight=2):
+        self.shapes = shapes
+        self.hbm_budget = hbm_budget
+        self.max_batch = max_batch or {sh: 64 for sh in shapes}
+        self.max_inflight = max_inflight
+
+    def run(self, requests):
+        import torch
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queues = {sh: collections.deque() for sh in self.shapes}
+        inflight = {sh: collections.deque() for sh in self.shapes}  # (event, [requests])
+        used = 0
+        peak_used = 0
+        batches = 0
+        done = []
+        start = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record()
+        while len(done) < len(requests):
+            now = time.perf_counter() - t0
+            # admission by HBM footprint, in arrival order
+            while pending and pending[0].arrival <= now and \
+                    used + pending[0].shape.footprint <= self.hbm_budget:
+                r = pending.popleft()
+                used += r.shape.footprint
+                queues[r.shape].append(r)
+            peak_used = max(peak_used, used)
+            # launch: every shape with room on the device takes its whole queue
+            for sh in self.shapes:
+                q = queues[sh]
+                if q and len(inflight[sh]) < self.max_inflight:
+                    take = [q.popleft() for _ in range(min(len(q), self.max_batch[sh]))]
+                    cp.hom_mul_gather(sh.ctx, [r.out for r in take], [r.ct1 for r in take],
+                                      [r.ct2 for r in take])
+                    ev = torch.cuda.Event(enable_timing=True)
+                    ev.record(sh.stream)
+                    for r in take:
+                        r.batch = batches
+                    batches += 1
+                    inflight[sh].append((ev, take))
+            # completions (batches of one stream finish in order)
+            progressed = False
+            for sh in self.shapes:
+                while inflight[sh] and inflight[sh][0][0].query():
+                    ev, take = inflight[sh].popleft()
+                    ms = start.elapsed_time(ev)
+                    for r in take:
+                        r.done_ms = ms
+                        used -= sh.footprint
+                        done.append(r)
+                    progressed = True
+            if not progressed and not any(queues[s] for s in self.shapes):
+                if pending and pending[0].arrival > time.perf_counter() - t0:
+                    time.sleep(min(50e-6, max(0.0, pending[0].arrival - (time.perf_counter() - t0))))
+        torch.cuda.synchronize()
+        lat = np.array([r.done_ms - 1e3 * r.arrival for r in done])
+        span_ms = max(r.done_ms for r in done) - 1e3 * min(r.arrival for r in done)
+        per_shape = {}
+        for sh in self.shapes:
+            ls = np.array([r.done_ms - 1e3 * r.arrival for r in done if r.shape is sh])
+            if len(ls):
+                per_shape[repr(sh)] = {"count": int(len(ls)), "p50_ms": float(np.percentile(ls, 50)),
+                                       "p99_ms": float(np.percentile(ls, 99))}
+        return {"requests": len(done), "batches": batches,
+                "req_per_s": len(done) / (span_ms * 1e-3),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+                "peak_admitted_bytes": int(peak_used), "per_shape": per_shape}

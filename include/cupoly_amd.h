@@ -155,6 +155,12 @@ int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi 
  * batch * 7 * (k+kl) * n/2 GIs (allocated on first use, grown as needed). */
 int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, const cupoly_gi *ct2,
                          int batch);
+/* Same computation for gathered operands (a serving batch of requests whose
+ * buffers are not contiguous): host arrays of `batch` device pointers, out[i]
+ * -> [3][k][n/2], ct1[i], ct2[i] -> [2][k][n/2].  The pointer tables are copied
+ * to the device on the context's stream before the call returns. */
+int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *const *ct1,
+                          const cupoly_gi *const *ct2, int batch);
 
 #ifdef __cplusplus
 }

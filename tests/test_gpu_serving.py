@@ -1,0 +1,37 @@
+"""Continuous batching: every request's result in the mixed, overlapped run is
+bit-identical to the same multiplication run alone (batch of one)."""
+import pytest
+import torch
+
+import cupoly_amd as cp
+from cupoly_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape_args,probs,rate", [
+    ([(2048, 2), (4096, 3)], [0.7, 0.3], 4000.0),
+    ([(8192, 3), (32768, 8)], [0.7, 0.3], 3000.0),
+])
+def test_serving_results_equal_alone(gpu, shape_args, probs, rate):
+    shapes = [serving.Shape(n, k) for n, k in shape_args]
+    reqs = serving.make_requests(shapes, probs, 40, rate, seed=5)
+    srv = serving.Server(shapes, hbm_budget=8 << 30, max_batch={sh: 8 for sh in shapes})
+    stats = srv.run(reqs)
+    assert stats["requests"] == 40
+    assert stats["batches"] >= 2
+    assert stats["p99_ms"] >= stats["p50_ms"] > 0
+    for r in reqs:
+        alone = cp.empty(r.out.shape)
+        cp.hom_mul_batch(r.shape.ctx, alone[None], r.ct1[None], r.ct2[None])
+        r.shape.ctx.sync()
+        assert torch.equal(alone, r.out), r.rid
+
+
+def test_admission_respects_hbm_budget(gpu):
+    shapes = [serving.Shape(2048, 2)]
+    reqs = serving.make_requests(shapes, [1.0], 30, 1e6, seed=6)
+    budget = 3 * shapes[0].footprint  # at most three requests admitted at once
+    stats = serving.Server(shapes, hbm_budget=budget).run(reqs)
+    assert stats["requests"] == 30
+    assert stats["peak_admitted_bytes"] <= budget
