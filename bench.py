@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--hm-batch", type=int, default=128,
                     help="ciphertext pairs per batch for the BFV multiplication core")
+    ap.add_argument("--enc-batch", type=int, default=1024,
+                    help="instances per batch of the fused a*s+e sampling step")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=20000.0,
                     help="offered load of the serving run, requests/s (Poisson)")
@@ -144,6 +146,31 @@ def main():
         del ct1, ct2, hout
         print("bench: hom-mul done %.0f hom-mults/s" % (HB / hm_s), file=sys.stderr, flush=True)
 
+    # sampling + encryption-shaped step: a*s + e, s ternary, e Gaussian (sigma 3.2)
+    enc = None
+    if args.enc_batch > 0:
+        EB = args.enc_batch
+        ea = cp.empty((EB, k, ng, 2))
+        for i in range(EB):
+            cp.Sampler.sample(ctx, ea[i], cp.UNIFORM, 7, i)  # a uniform, DGT domain
+        eout = cp.empty((EB, k, ng, 2))
+        cp.sample_muladd_batch(ctx, eout, ea, 31, 0)
+        torch.cuda.synchronize()
+        ereps = 5
+        t4 = time.perf_counter()
+        for _ in range(ereps):
+            cp.sample_muladd_batch(ctx, eout, ea, 31, 0)
+        torch.cuda.synchronize()
+        es = (time.perf_counter() - t4) / ereps
+        ebytes = 16 * k * n * EB
+        enc = {"value": EB / es, "unit": "a*s+e/s", "batch": EB, "ms_per_batch": es * 1e3,
+               "workload": "a*s+e in the DGT domain, s ternary, e discrete Gaussian sigma=3.2 "
+                           "|e|<=19, Philox counters, n=%d k=%d" % (n, k),
+               "algorithmic_bytes": ebytes, "achieved_GBs": ebytes / es / 1e9,
+               "hbm_frac": ebytes / es / 1e9 / HBM_PEAK_GBS}
+        del ea, eout
+        print("bench: a*s+e done %.0f /s" % (EB / es), file=sys.stderr, flush=True)
+
     # serving: Poisson arrivals, 70% (n=2^13, k=3) / 30% (n=2^15, k=8) BFV
     # multiplications, continuous batching, HBM-footprint admission
     serve = None
@@ -213,6 +240,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "hom_mul": hm,
+        "sample_muladd": enc,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

@@ -68,6 +68,7 @@ SIGNATURES = {
     "cupoly_polymul_plain": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
 }
 
 _lib = None
@@ -350,6 +351,16 @@ def hom_mul_gather(ctx, outs, ct1s, ct2s):
     p1 = arr(*[_ptr(t).value for t in ct1s])
     p2 = arr(*[_ptr(t).value for t in ct2s])
     _check(lib().cupoly_hom_mul_gather(ctx.handle, po, p1, p2, b))
+
+
+def sample_muladd_batch(ctx, out, a, seed, stream0):
+    """out[i] = a[i] * s_i + e_i (DGT domain, base Q), s_i ternary and e_i
+    discrete Gaussian drawn on the device; a, out [batch, k, N, 2]."""
+    e = ctx.engine
+    batch = a.shape[0]
+    _need(a, e.k, e.N, batch)
+    _need(out, e.k, e.N, batch)
+    _check(lib().cupoly_sample_muladd_batch(ctx.handle, _ptr(out), _ptr(a), batch, seed, stream0))
 
 
 # ---- host <-> device helpers ------------------------------------------------

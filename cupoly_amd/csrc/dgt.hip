@@ -25,332 +25,7 @@
 //
 // Sizes: N = 16..8192 in one kernel; N = 16384, 32768 in two kernels
 // (stages [0,S) on strided columns, then [S, log N) on contiguous blocks).
-#include "internal.hpp"
-
-#define EPT 8       // Gaussian slots per thread
-#ifndef CP_STAGE_BARRIER
-#define CP_STAGE_BARRIER 1
-#endif
-#define LOG_EPT 3
-
-// -------------------------------------------------------------------------
-// Index maps: local slot index (inside the workgroup's LDS array) -> global
-// slot index inside one residue polynomial.
-// -------------------------------------------------------------------------
-// kAligned: global index == local index modulo 8 (true for whole polynomials
-// and 128-slot blocks), so in a last pass of unit stride 1 the twiddles of the
-// final three stages are known at compile time.
-struct MapIdentity {  // whole polynomial, local == global
-  static constexpr bool kAligned = true;
-  __device__ __forceinline__ int operator()(int l) const { return l; }
-};
-template <int LOGC, int LOGNS>  // 2^LOGC columns of a polynomial, column stride 2^LOGNS
-struct MapColumns {
-  static constexpr bool kAligned = false;
-  int col0;
-  __device__ __forceinline__ int operator()(int l) const {
-    return col0 + (l & ((1 << LOGC) - 1)) + ((l >> LOGC) << LOGNS);
-  }
-};
-struct MapBlock {  // contiguous block starting at base (a multiple of 8)
-  static constexpr bool kAligned = true;
-  int base;
-  __device__ __forceinline__ int operator()(int l) const { return base + l; }
-};
-
-__device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, int e, uint64_t &w,
-                                      uint64_t &wp) {
-  const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(tw + 2 * e);
-  w = v.x;
-  wp = v.y;
-}
-
-// One forward pass over local stages [LS0, LS0+R) of a 2^LOGM local array.
-// Global stage = local stage + SOFF; LOGN = log2 of the global transform size.
-// Butterflies whose twiddle is 1 are done without a multiply: always in the
-// last global stage, and, when the unit stride is 1 on an aligned map, every
-// butterfly whose index within its group is 0.
-template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
-__device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
-                                         uint64_t p, const Map &map) {
-  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
-#pragma unroll
-  for (int ju = 0; ju < NU; ju++) {
-    const int u = t + ju * T;
-    const int base = (u / ST) * (U * ST) + (u % ST);
-#pragma unroll
-    for (int h = 0; h < R; h++) {
-      const int half = U >> (h + 1);
-      const int s = LS0 + h + SOFF;
-      const int m = 1 << (LOGN - 1 - s);
-#pragma unroll
-      for (int q = 0; q < U; q++) {
-        if (q & half) continue;
-        cp_gi &A = x[ju * U + q];
-        cp_gi &B = x[ju * U + q + half];
-        const uint64_t sr = A.re + B.re, si = A.imag + B.imag;
-        const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
-        A.re = cp_red2(sr, p);
-        A.imag = cp_red2(si, p);
-        const bool static_j = Map::kAligned && ST == 1 && m <= U;
-        const int jc = q & (m - 1);
-        if (s == LOGN - 1 || (static_j && jc == 0)) {  // twiddle 1
-          B.re = cp_red2(dr, p);
-          B.imag = cp_red2(di, p);
-        } else {
-          const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
-          uint64_t w, wp;
-          ld_tw(tw, e, w, wp);
-          B.re = cp_shoup(dr, w, wp, p);
-          B.imag = cp_shoup(di, w, wp, p);
-        }
-      }
-      if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-}
-
-// One inverse pass: same units, stages in reverse, DIT butterflies with omega^-1.
-template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
-__device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
-                                         uint64_t p, const Map &map) {
-  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
-#pragma unroll
-  for (int ju = 0; ju < NU; ju++) {
-    const int u = t + ju * T;
-    const int base = (u / ST) * (U * ST) + (u % ST);
-#pragma unroll
-    for (int h = R - 1; h >= 0; h--) {
-      const int half = U >> (h + 1);
-      const int s = LS0 + h + SOFF;
-      const int m = 1 << (LOGN - 1 - s);
-#pragma unroll
-      for (int q = 0; q < U; q++) {
-        if (q & half) continue;
-        cp_gi &A = x[ju * U + q];
-        cp_gi &B = x[ju * U + q + half];
-        uint64_t tr, ti;
-        const bool static_j = Map::kAligned && ST == 1 && m <= U;
-        const int jc = q & (m - 1);
-        if (s == LOGN - 1 || (static_j && jc == 0)) {
-          tr = B.re;
-          ti = B.imag;
-        } else {
-          const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
-          uint64_t w, wp;
-          ld_tw(tw, e, w, wp);
-          tr = cp_shoup(B.re, w, wp, p);
-          ti = cp_shoup(B.imag, w, wp, p);
-        }
-        B.re = cp_red2(A.re + p - tr, p);
-        B.imag = cp_red2(A.imag + p - ti, p);
-        A.re = cp_red2(A.re + tr, p);
-        A.imag = cp_red2(A.imag + ti, p);
-      }
-      if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-}
-
-// Local index of register slot i in the layout of the pass starting at LS0 with R stages.
-template <int LOGM, int LS0, int R>
-__device__ __forceinline__ int pass_index(int t, int i) {
-  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R);
-  const int ju = i / U, q = i % U;
-  const int u = t + ju * T;
-  return (u / ST) * (U * ST) + (u % ST) + q * ST;
-}
-
-// Same for a full radix-8 pass (R = 3, one unit per thread) starting at a
-// runtime local stage ls0: unit stride ST = M >> (ls0 + 3).
-template <int LOGM>
-__device__ __forceinline__ int pass_index_rt(int t, int i, int ls0) {
-  const int lst = LOGM - ls0 - LOG_EPT;  // log2(ST)
-  return ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1)) + (i << lst);
-}
-
-template <int LOGM, int LS0, int R>
-__device__ __forceinline__ void lds_store(cp_gi *lds, const cp_gi (&x)[EPT], int t) {
-#pragma unroll
-  for (int i = 0; i < EPT; i++) lds[pass_index<LOGM, LS0, R>(t, i)] = x[i];
-}
-template <int LOGM, int LS0, int R>
-__device__ __forceinline__ void lds_load(const cp_gi *lds, cp_gi (&x)[EPT], int t) {
-#pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index<LOGM, LS0, R>(t, i)];
-}
-template <int LOGM>
-__device__ __forceinline__ void lds_store_rt(cp_gi *lds, const cp_gi (&x)[EPT], int t, int ls0) {
-#pragma unroll
-  for (int i = 0; i < EPT; i++) lds[pass_index_rt<LOGM>(t, i, ls0)] = x[i];
-}
-template <int LOGM>
-__device__ __forceinline__ void lds_load_rt(const cp_gi *lds, cp_gi (&x)[EPT], int t, int ls0) {
-#pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index_rt<LOGM>(t, i, ls0)];
-}
-
-// Full radix-8 pass at a runtime local stage ls0 (never the last global
-// stage, so every twiddle is a real multiply).
-template <int LOGN, int LOGM, int SOFF, class Map>
-__device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
-                                            uint64_t p, const Map &map, int ls0) {
-  const int lst = LOGM - ls0 - LOG_EPT;
-  const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
-#pragma unroll
-  for (int h = 0; h < LOG_EPT; h++) {
-    const int half = EPT >> (h + 1);
-    const int s = ls0 + h + SOFF;
-    const int mmask = (1 << (LOGN - 1 - s)) - 1;
-#pragma unroll
-    for (int q = 0; q < EPT; q++) {
-      if (q & half) continue;
-      cp_gi &A = x[q];
-      cp_gi &B = x[q + half];
-      const uint64_t sr = A.re + B.re, si = A.imag + B.imag;
-      const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
-      A.re = cp_red2(sr, p);
-      A.imag = cp_red2(si, p);
-      const int e = (map(base + (q << lst)) & mmask) << s;
-      uint64_t w, wp;
-      ld_tw(tw, e, w, wp);
-      B.re = cp_shoup(dr, w, wp, p);
-      B.imag = cp_shoup(di, w, wp, p);
-    }
-    // one stage's twiddles in flight at a time (bounds VGPR use)
-    if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-template <int LOGN, int LOGM, int SOFF, class Map>
-__device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
-                                            uint64_t p, const Map &map, int ls0) {
-  const int lst = LOGM - ls0 - LOG_EPT;
-  const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
-#pragma unroll
-  for (int h = LOG_EPT - 1; h >= 0; h--) {
-    const int half = EPT >> (h + 1);
-    const int s = ls0 + h + SOFF;
-    const int mmask = (1 << (LOGN - 1 - s)) - 1;
-#pragma unroll
-    for (int q = 0; q < EPT; q++) {
-      if (q & half) continue;
-      cp_gi &A = x[q];
-      cp_gi &B = x[q + half];
-      const int e = (map(base + (q << lst)) & mmask) << s;
-      uint64_t w, wp;
-      ld_tw(tw, e, w, wp);
-      const uint64_t tr = cp_shoup(B.re, w, wp, p);
-      const uint64_t ti = cp_shoup(B.imag, w, wp, p);
-      B.re = cp_red2(A.re + p - tr, p);
-      B.imag = cp_red2(A.imag + p - ti, p);
-      A.re = cp_red2(A.re + tr, p);
-      A.imag = cp_red2(A.imag + ti, p);
-    }
-    if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-constexpr int cmin(int a, int b) { return a < b ? a : b; }
-
-// Pass plan over local stages [LSB, LSE): an optional partial pass of
-// R0 = NS mod 3 stages first, then NF full radix-8 passes.  The full passes
-// but the last run in a loop that is not unrolled (so the compiler cannot
-// hoist twiddle loads across passes and spill); the last pass is compiled
-// with its stages known, which lets its trivial twiddles drop out.
-template <int LSB, int LSE>
-struct PassPlan {
-  static constexpr int NS = LSE - LSB;
-  static constexpr int R0 = NS % LOG_EPT;                  // partial first pass (0: none)
-  static constexpr int NF = NS / LOG_EPT;                  // full passes
-  static constexpr int LSF = LSB + R0;                     // first stage of the full passes
-  static constexpr int LSL = NF ? LSE - LOG_EPT : LSB;     // first stage of the last pass
-  static constexpr int RL = NF ? LOG_EPT : R0;             // stages in the last pass
-  static constexpr int LS_FIRST = LSB;                     // first pass
-  static constexpr int R_FIRST = R0 ? R0 : LOG_EPT;
-};
-
-// Register layout of the first pass (used for direct global I/O).
-template <int LOGM, int LSB, int LSE>
-__device__ __forceinline__ int first_index(int t, int i) {
-  using PP = PassPlan<LSB, LSE>;
-  return pass_index<LOGM, PP::LS_FIRST, PP::R_FIRST>(t, i);
-}
-
-// Forward passes over local stages [LSB, LSE), registers starting in the
-// first-pass layout and ending in the last-pass layout.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
-__device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
-                                           const uint64_t *__restrict__ tw, uint64_t p,
-                                           const Map &map) {
-  using PP = PassPlan<LSB, LSE>;
-  if constexpr (PP::R0 > 0) {
-    dif_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
-    if constexpr (PP::NF == 0) return;
-    __syncthreads();
-    lds_store<LOGM, LSB, PP::R0>(lds, x, t);
-    __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, PP::LSF);
-  }
-#pragma nounroll
-  for (int i = 0; i < PP::NF - 1; i++) {
-    const int ls0 = PP::LSF + LOG_EPT * i;
-    dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
-    __syncthreads();
-    lds_store_rt<LOGM>(lds, x, t, ls0);
-    __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
-  }
-  if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
-}
-
-// Inverse passes over local stages [LSB, LSE) in reverse order; registers
-// start in the last-pass layout and end in the first-pass layout.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
-__device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
-                                           const uint64_t *__restrict__ tw, uint64_t p,
-                                           const Map &map) {
-  using PP = PassPlan<LSB, LSE>;
-  if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
-#pragma nounroll
-  for (int i = PP::NF - 2; i >= 0; i--) {
-    const int ls0 = PP::LSF + LOG_EPT * i;
-    __syncthreads();
-    lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
-    __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, ls0);
-    dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
-  }
-  if constexpr (PP::R0 > 0) {
-    if constexpr (PP::NF > 0) {
-      __syncthreads();
-      lds_store_rt<LOGM>(lds, x, t, PP::LSF);
-      __syncthreads();
-      lds_load<LOGM, LSB, PP::R0>(lds, x, t);
-    }
-    dit_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
-  }
-}
-
-// Layout of the last pass over [LSB, LSE).
-template <int LOGM, int LSB, int LSE>
-struct LastPass {
-  static constexpr int LS0 = PassPlan<LSB, LSE>::LSL;
-  static constexpr int RL = PassPlan<LSB, LSE>::RL;
-};
-
-// natural layout index of register i: t + i*T (LDS-staged global I/O)
-template <int LOGM>
-__device__ __forceinline__ int nat_index(int t, int i) {
-  return t + i * ((1 << LOGM) / EPT);
-}
-
-__device__ __forceinline__ void twist(cp_gi &v, const uint64_t *__restrict__ tab, int g, uint64_t p) {
-  const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(tab + 6 * (size_t)g);
-  const ulonglong2 a = k2[0], b = k2[1], c = k2[2];
-  const uint64_t k[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
-  v = cp_gimul_const(v, k, p);
-}
+#include "dgt_dev.hpp"
 
 // -------------------------------------------------------------------------
 // Single-kernel transform: N = 2^LOGN <= 8192.  ROWS residue polynomials per
@@ -631,10 +306,6 @@ __global__ void __launch_bounds__(1024)
 // -------------------------------------------------------------------------
 // Host side
 // -------------------------------------------------------------------------
-template <int LOGN>
-constexpr int rows_for() {
-  return ((1 << LOGN) / EPT) >= 256 ? 1 : 256 / ((1 << LOGN) / EPT);
-}
 
 template <class K>
 static int set_lds(K kernel, size_t bytes) {

@@ -1,0 +1,132 @@
+// Fused sampling + multiply-add: out = a * s + e in the DGT domain.
+//
+// The encryption-shaped step of RLWE schemes: the reference samples s and e
+// into poly_t objects (Sampler::sample, src/cuda/sampler.cu:576-635, each
+// followed by a forward DGT) and then calls poly_mul_add (polynomial.cu:916-948
+// -> muladd_dgt, dgt.cu:1439-1499): five launches and four polynomials through
+// HBM.  Here one workgroup per (instance, residue) draws s (ternary) and e
+// (discrete Gaussian) from their Philox counters, twists and transforms both
+// in LDS, multiplies by a (read once, in the DGT domain) and writes out: the
+// HBM traffic is a in and out out, 16 k n bytes per instance.
+//
+// Instance i uses Philox stream stream0 + 2i for s and stream0 + 2i + 1 for e,
+// so the small polynomials are the ones cupoly_sample_coeff draws for those
+// streams (same values in every residue).
+#include "dgt_dev.hpp"
+#include "philox.hpp"
+
+template <int LOGN, int ROWS>
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+    k_sample_muladd(cp_gi *__restrict__ out, const cp_gi *__restrict__ a, int nunits, int nres,
+                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                    const uint64_t *__restrict__ tsf_all, uint64_t seed, uint64_t stream0,
+                    cp_cdt cdt) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  using LP = LastPass<LOGN, 0, LOGN>;
+  extern __shared__ cp_gi lds_all[];
+  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
+  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
+  cp_gi *lds = lds_all + row * N;
+  int unit = blockIdx.x * ROWS + row;
+  const bool active = unit < nunits;
+  if (!active) unit = nunits - 1;
+  int rid = unit % nres;
+  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
+  const uint64_t inst = (uint64_t)(unit / nres);
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const uint64_t *twf = twf_all + (size_t)rid * N;
+  const uint64_t *tsf = tsf_all + (size_t)rid * N * 6;
+  const size_t off = (size_t)unit * N;
+  MapIdentity id;
+  cp_gi x[EPT], y[EPT];
+  // s: ternary, coefficient g in re and g + N in imag of slot g
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<LOGN, 0, LOGN>(t, i);
+    x[i].re = small_mod(ternary_coef(seed, stream0 + 2 * inst, (uint32_t)g), p);
+    x[i].imag = small_mod(ternary_coef(seed, stream0 + 2 * inst, (uint32_t)(g + N)), p);
+    twist(x[i], tsf, g, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
+  // a * s at the spectrum positions this thread holds (last-pass layout)
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+    x[i] = cp_gimul(a[off + pass_index<LOGN, LP::LS0, LP::RL>(t, i)], x[i], pr);
+  // e: discrete Gaussian
+  int te = t;
+  asm volatile("" : "+v"(te));
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<LOGN, 0, LOGN>(te, i);
+    y[i].re = small_mod(gaussian_coef(seed, stream0 + 2 * inst + 1, (uint32_t)g, cdt), p);
+    y[i].imag = small_mod(gaussian_coef(seed, stream0 + 2 * inst + 1, (uint32_t)(g + N), cdt), p);
+    twist(y[i], tsf, g, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(y, lds, t, twf, p, id);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      cp_gi r;
+      r.re = cp_add(x[i].re, y[i].re, p);
+      r.imag = cp_add(x[i].imag, y[i].imag, p);
+      out[off + pass_index<LOGN, LP::LS0, LP::RL>(te, i)] = r;
+    }
+  }
+}
+
+template <int LOGN>
+static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch, uint64_t seed,
+                           uint64_t stream0) {
+  cupoly_engine *e = c->e;
+  const int nres = e->k, nunits = nres * batch;
+  constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
+  const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+  static bool attr_set = false;
+  if (!attr_set && lds > 64 * 1024) {
+    CP_HIP(hipFuncSetAttribute((const void *)k_sample_muladd<LOGN, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_sample_muladd<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
+                     c->stream, out, a, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_twist_fwd,
+                     seed, stream0, cp_make_cdt());
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
+extern "C" int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
+                                          int batch, uint64_t seed, uint64_t stream0) {
+  if (!c || !out || !a || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "sample_muladd: bad argument");
+  if (batch == 0) return CUPOLY_OK;
+  switch (c->e->logng) {
+    case 4: return sample_muladd_t<4>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 5: return sample_muladd_t<5>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 6: return sample_muladd_t<6>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 7: return sample_muladd_t<7>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 8: return sample_muladd_t<8>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 9: return sample_muladd_t<9>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 10: return sample_muladd_t<10>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 11: return sample_muladd_t<11>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 12: return sample_muladd_t<12>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    case 13: return sample_muladd_t<13>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
+    default: break;
+  }
+  // n > 16384: sample, transform and multiply-add as separate launches
+  cupoly_engine *e = c->e;
+  const size_t stride = (size_t)e->k * e->ng;
+  void *ws;
+  int rc = cp_workspace(c, 2 * stride * sizeof(cp_gi), &ws);
+  if (rc) return rc;
+  cupoly_gi *s = (cupoly_gi *)ws, *ee = s + stride;
+  for (int i = 0; i < batch; i++) {
+    if ((rc = cupoly_sample(c, s, CUPOLY_TERNARY, seed, stream0 + 2 * (uint64_t)i, CUPOLY_QBASE)) ||
+        (rc = cupoly_sample(c, ee, CUPOLY_DISCRETE_GAUSSIAN, seed, stream0 + 2 * (uint64_t)i + 1,
+                           CUPOLY_QBASE)) ||
+        (rc = cupoly_muladd_dgt(c, out + i * stride, a + i * stride, s, ee, CUPOLY_QBASE)))
+      return rc;
+  }
+  return CUPOLY_OK;
+}

@@ -162,6 +162,14 @@ int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, co
 int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *const *ct1,
                           const cupoly_gi *const *ct2, int batch);
 
+/* Encryption-shaped step for a batch: out[i] = a[i] * s_i + e_i in the DGT
+ * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian
+ * (stream stream0 + 2i + 1), both drawn on the device (Sampler::sample x2 +
+ * poly_mul_add, sampler.cu:576-635, polynomial.cu:916-948); a, out
+ * [batch][k][n/2].  One fused launch for n <= 16384. */
+int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, int batch,
+                               uint64_t seed, uint64_t stream0);
+
 #ifdef __cplusplus
 }
 #endif

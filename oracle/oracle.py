@@ -326,3 +326,15 @@ def hom_mul(P, ct1, ct2):
         zc = dgt(P, z, 2, 1)
         out.append(basis_ext_b_to_q(P, complex_scaling(P, zc[:k], zc[k:])))
     return np.stack(out)
+
+
+def sample_muladd(P, a, seed, stream0):
+    """a GI [batch, k, Ng, 2] (DGT domain) -> a*s + e with s ternary (stream
+    stream0 + 2i) and e discrete Gaussian (stream0 + 2i + 1), the reference's
+    Sampler::sample + poly_mul_add (sampler.cu:576-635, polynomial.cu:916-948)."""
+    out = np.empty_like(u64(a))
+    for i in range(a.shape[0]):
+        s = fold(small_to_residues(sample_small(P.n, 2, seed, stream0 + 2 * i), P.q), P.Ng)
+        e = fold(small_to_residues(sample_small(P.n, 3, seed, stream0 + 2 * i + 1), P.q), P.Ng)
+        out[i] = pointwise(P, a[i], dgt(P, s, 0, 0), dgt(P, e, 0, 0), base=0, op=3)
+    return out

@@ -228,3 +228,16 @@ def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
     h = cp.to_host(out)
     for i in range(batch):
         assert np.array_equal(h[i], O.hom_mul(P, ct1[i], ct2[i])), i
+
+
+@pytest.mark.parametrize("k,n,batch", [(2, 2048, 5), (4, 16384, 3), (3, 32768, 2)])
+def test_sample_muladd_matches_oracle(gpu, k, n, batch):
+    """Config-4 shape (n = 2^14, k = 4): a * s + e with s ternary and e discrete
+    Gaussian drawn on the device, bit-exact against the oracle on the same
+    Philox counters."""
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 300 + k)
+    out = cp.empty(a.shape)
+    cp.sample_muladd_batch(ctx, out, cp.to_device(a), 4242, 10)
+    assert np.array_equal(cp.to_host(out), O.sample_muladd(P, a, 4242, 10))
