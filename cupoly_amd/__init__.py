@@ -69,6 +69,15 @@ SIGNATURES = {
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
+    "cupoly_uneven_create": ([_vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _int,
+                              ctypes.POINTER(_vp)], _int),
+    "cupoly_uneven_run": ([_vp], _int),
+    "cupoly_uneven_capture": ([_vp], _int),
+    "cupoly_uneven_run_plain": ([_vp], _int),
+    "cupoly_uneven_set_tuning": ([_vp, _int, _int], _int),
+    "cupoly_uneven_get_tuning": ([_vp, _vp, _vp, _vp], _int),
+    "cupoly_uneven_autotune": ([_vp, _int, _vp, _vp, _vp], _int),
+    "cupoly_uneven_destroy": ([_vp], _int),
 }
 
 _lib = None
@@ -361,6 +370,65 @@ def sample_muladd_batch(ctx, out, a, seed, stream0):
     _need(a, e.k, e.N, batch)
     _need(out, e.k, e.N, batch)
     _check(lib().cupoly_sample_muladd_batch(ctx.handle, _ptr(out), _ptr(a), batch, seed, stream0))
+
+
+class UnevenPlan:
+    """Persistent one-launch batch of products of mixed ring degree and width.
+    items: list of (n, k, out, a, b) with device tensors [k, n/2, 2];
+    engines: CUDAEngine per ring degree used."""
+
+    def __init__(self, ctx, engines, items):
+        self.ctx, self.engines, self.items = ctx, engines, items
+        cnt = len(items)
+        for n, k, o, a, b in items:
+            for t in (o, a, b):
+                if tuple(t.shape) != (k, n // 2, 2):
+                    raise ValueError("cupoly_amd: uneven operand shape %s" % (tuple(t.shape),))
+        arrp = ctypes.c_void_p * max(cnt, 1)
+        arri = ctypes.c_int * max(cnt, 1)
+        eng = (ctypes.c_void_p * len(engines))(*[e.handle.value for e in engines])
+        h = ctypes.c_void_p()
+        _check(lib().cupoly_uneven_create(
+            ctx.handle, eng, len(engines), arri(*[it[0] for it in items]),
+            arri(*[it[1] for it in items]), arrp(*[it[2].data_ptr() for it in items]),
+            arrp(*[it[3].data_ptr() for it in items]), arrp(*[it[4].data_ptr() for it in items]),
+            cnt, ctypes.byref(h)))
+        self.handle = h
+
+    def run(self):
+        _check(lib().cupoly_uneven_run(self.handle))
+
+    def capture(self):
+        _check(lib().cupoly_uneven_capture(self.handle))
+
+    def run_plain(self):
+        _check(lib().cupoly_uneven_run_plain(self.handle))
+
+    def set_tuning(self, grid, order):
+        _check(lib().cupoly_uneven_set_tuning(self.handle, grid, order))
+
+    def tuning(self):
+        g, o, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().cupoly_uneven_get_tuning(self.handle, ctypes.byref(g), ctypes.byref(o),
+                                              ctypes.byref(nt)))
+        return {"grid": g.value, "order": o.value, "tiles": nt.value}
+
+    def autotune(self, reps=3):
+        g, o, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _check(lib().cupoly_uneven_autotune(self.handle, reps, ctypes.byref(g), ctypes.byref(o),
+                                            ctypes.byref(ms)))
+        return {"grid": g.value, "order": o.value, "ms": ms.value}
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().cupoly_uneven_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 # ---- host <-> device helpers ------------------------------------------------

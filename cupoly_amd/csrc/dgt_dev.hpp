@@ -42,14 +42,32 @@ __device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, int e, ui
   w = v.x;
   wp = v.y;
 }
+// Pointers typed in the global address space: code reached through an
+// out-of-line call (or pointers read from memory) would otherwise address
+// tables with flat instructions.
+#define CP_GLOBAL __attribute__((address_space(1)))
+typedef unsigned long long cp_u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cp_gi ld_gi(const CP_GLOBAL cp_gi *ptr) {
+  const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)ptr;
+  return cp_gi{v.x, v.y};
+}
+__device__ __forceinline__ void st_gi(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
+  *(CP_GLOBAL cp_u64x2 *)ptr = cp_u64x2{v.re, v.imag};
+}
+__device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw, int e,
+                                      uint64_t &w, uint64_t &wp) {
+  const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)(tw + 2 * e);
+  w = v.x;
+  wp = v.y;
+}
 
 // One forward pass over local stages [LS0, LS0+R) of a 2^LOGM local array.
 // Global stage = local stage + SOFF; LOGN = log2 of the global transform size.
 // Butterflies whose twiddle is 1 are done without a multiply: always in the
 // last global stage, and, when the unit stride is 1 on an aligned map, every
 // butterfly whose index within its group is 0.
-template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
-__device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map, class TW>
+__device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, TW __restrict__ tw,
                                          uint64_t p, const Map &map) {
   constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
 #pragma unroll
@@ -89,8 +107,8 @@ __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, const uint64_t 
 }
 
 // One inverse pass: same units, stages in reverse, DIT butterflies with omega^-1.
-template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map>
-__device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map, class TW>
+__device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, TW __restrict__ tw,
                                          uint64_t p, const Map &map) {
   constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
 #pragma unroll
@@ -170,8 +188,8 @@ __device__ __forceinline__ void lds_load_rt(const cp_gi *lds, cp_gi (&x)[EPT], i
 
 // Full radix-8 pass at a runtime local stage ls0 (never the last global
 // stage, so every twiddle is a real multiply).
-template <int LOGN, int LOGM, int SOFF, class Map>
-__device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+template <int LOGN, int LOGM, int SOFF, class Map, class TW>
+__device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, TW __restrict__ tw,
                                             uint64_t p, const Map &map, int ls0) {
   const int lst = LOGM - ls0 - LOG_EPT;
   const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
@@ -200,8 +218,8 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, const uint64
   }
 }
 
-template <int LOGN, int LOGM, int SOFF, class Map>
-__device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, const uint64_t *__restrict__ tw,
+template <int LOGN, int LOGM, int SOFF, class Map, class TW>
+__device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, TW __restrict__ tw,
                                             uint64_t p, const Map &map, int ls0) {
   const int lst = LOGM - ls0 - LOG_EPT;
   const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
@@ -257,9 +275,9 @@ __device__ __forceinline__ int first_index(int t, int i) {
 
 // Forward passes over local stages [LSB, LSE), registers starting in the
 // first-pass layout and ending in the last-pass layout.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW>
 __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
-                                           const uint64_t *__restrict__ tw, uint64_t p,
+                                           TW __restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
   if constexpr (PP::R0 > 0) {
@@ -284,9 +302,9 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
 
 // Inverse passes over local stages [LSB, LSE) in reverse order; registers
 // start in the last-pass layout and end in the first-pass layout.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map>
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW>
 __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
-                                           const uint64_t *__restrict__ tw, uint64_t p,
+                                           TW __restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
@@ -326,6 +344,13 @@ __device__ __forceinline__ int nat_index(int t, int i) {
 __device__ __forceinline__ void twist(cp_gi &v, const uint64_t *__restrict__ tab, int g, uint64_t p) {
   const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(tab + 6 * (size_t)g);
   const ulonglong2 a = k2[0], b = k2[1], c = k2[2];
+  const uint64_t k[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+  v = cp_gimul_const(v, k, p);
+}
+__device__ __forceinline__ void twist(cp_gi &v, const CP_GLOBAL uint64_t *__restrict__ tab, int g,
+                                      uint64_t p) {
+  const CP_GLOBAL cp_u64x2 *k2 = (const CP_GLOBAL cp_u64x2 *)(tab + 6 * (size_t)g);
+  const cp_u64x2 a = k2[0], b = k2[1], c = k2[2];
   const uint64_t k[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
   v = cp_gimul_const(v, k, p);
 }

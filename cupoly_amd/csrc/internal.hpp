@@ -85,3 +85,4 @@ int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int
 int cp_polymul_fused_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch);
 int cp_dgt_setup(cupoly_engine *e);
 int cp_workspace(cupoly_ctx *c, size_t bytes, void **out);
+int cp_pointwise_mul_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nres);

@@ -116,6 +116,15 @@ static int launch_pw(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cu
   return CUPOLY_OK;
 }
 
+int cp_pointwise_mul_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nres) {
+  const long total = (long)c->e->ng * nres;
+  hipLaunchKernelGGL(k_pointwise<PW_MUL>, dim3(grid_for(total)), dim3(256), 0, c->stream, out, a, b,
+                     (const cp_gi *)nullptr, (const uint64_t *)nullptr, (uint64_t)0, c->e->ng, nres,
+                     0, c->e->d_primes);
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
 extern "C" int cupoly_add_dgt(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
                               const cupoly_gi *b, int base) {
   if (!b) return cp_fail(CUPOLY_ERR_ARG, "add_dgt: null b");

@@ -26,6 +26,7 @@ typedef struct { uint64_t re, imag; } cupoly_gi; /* GaussianInteger, settings.h:
 
 typedef struct cupoly_engine cupoly_engine;     /* CUDAEngine + DGTEngine state */
 typedef struct cupoly_ctx cupoly_ctx;           /* Context (stream + scratch)   */
+typedef struct cupoly_uneven cupoly_uneven;     /* plan for an uneven batch      */
 
 /* Error codes (the reference asserts / throws std::runtime_error / exit(1)). */
 enum {
@@ -169,6 +170,32 @@ int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi 
  * [batch][k][n/2].  One fused launch for n <= 16384. */
 int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, int batch,
                                uint64_t seed, uint64_t stream0);
+
+/* ---- uneven batches (no reference counterpart: the reference holds one
+ * (n, k) per process, cudaengine.h:71-93) -----------------------------------
+ * `count` independent products out[i] = a[i] * b[i] mod (X^n[i] + 1), residues
+ * 0..k[i]-1, coefficient domain, [k[i]][n[i]/2] GIs each, n[i] in 2^11..2^15.
+ * engines[] holds one engine per ring degree used (k of each >= the largest
+ * k[i] of that degree); the plan keeps device copies of the work list and runs
+ * on c's stream.  Pointers must stay valid for the plan's lifetime. */
+int cupoly_uneven_create(cupoly_ctx *c, cupoly_engine *const *engines, int n_engines, const int *n,
+                         const int *k, cupoly_gi *const *out, const cupoly_gi *const *a,
+                         const cupoly_gi *const *b, int count, cupoly_uneven **plan);
+/* One persistent launch (plus a counter-reset kernel), or the captured graph. */
+int cupoly_uneven_run(cupoly_uneven *plan);
+/* Capture reset + persistent kernel into a hipGraph; later runs replay it. */
+int cupoly_uneven_capture(cupoly_uneven *plan);
+/* The same products one launch per op (forward a, forward b, mul, inverse),
+ * product by product; a and b are left in the DGT domain. */
+int cupoly_uneven_run_plain(cupoly_uneven *plan);
+/* Tunables: persistent grid size and tile order (0: most expensive first,
+ * 1: input order).  autotune times every candidate `reps` times on the plan's
+ * own batch and keeps the fastest. */
+int cupoly_uneven_set_tuning(cupoly_uneven *plan, int grid, int order);
+int cupoly_uneven_get_tuning(const cupoly_uneven *plan, int *grid, int *order, int *ntiles);
+int cupoly_uneven_autotune(cupoly_uneven *plan, int reps, int *best_grid, int *best_order,
+                           float *best_ms);
+int cupoly_uneven_destroy(cupoly_uneven *plan);
 
 #ifdef __cplusplus
 }
