@@ -238,6 +238,36 @@ def test_oracle_dgt_matches_reference_hierarchical_kernels(Ng, rid):
     assert [tuple(int(v) for v in row) for row in x] == A
 
 
+def test_pointwise_matches_gaussian_integer_arithmetic():
+    """or_pointwise (add, sub, GI product, GI multiply-add) against Python
+    integers on random slots and the edge residues 0, 1, p - 1."""
+    P = O.Params(3, 2, 64, 65537)
+    rng = random.Random(21)
+    Ng = P.Ng
+    shape = (len(P.primes), Ng, 2)
+    vals = {}
+    for name in "abc":
+        x = np.empty(shape, dtype=np.uint64)
+        for r, p in enumerate(P.primes):
+            edge = [0, 1, p - 1]
+            for i in range(Ng):
+                for j in range(2):
+                    x[r, i, j] = edge[(i + j) % 3] if i < 6 else rng.randrange(p)
+        vals[name] = x
+    a, b, c = vals["a"], vals["b"], vals["c"]
+    got = [O.pointwise(P, a, b, c, base=2, op=op) for op in range(4)]
+    for r, p in enumerate(P.primes):
+        for i in range(Ng):
+            ar, ai = int(a[r, i, 0]), int(a[r, i, 1])
+            br, bi = int(b[r, i, 0]), int(b[r, i, 1])
+            cr, ci = int(c[r, i, 0]), int(c[r, i, 1])
+            pr_, pi_ = (ar * br - ai * bi) % p, (ar * bi + ai * br) % p
+            want = [((ar + br) % p, (ai + bi) % p), ((ar - br) % p, (ai - bi) % p),
+                    (pr_, pi_), ((pr_ + cr) % p, (pi_ + ci) % p)]
+            for op in range(4):
+                assert (int(got[op][r, i, 0]), int(got[op][r, i, 1])) == want[op], (op, r, i)
+
+
 def test_dgt_roundtrip_and_constant():
     P = O.Params(3, 4, 4096, 65537)
     rng = np.random.default_rng(3)
