@@ -20,25 +20,30 @@
 // final three stages are known at compile time.
 struct MapIdentity {  // whole polynomial, local == global
   static constexpr bool kAligned = true;
+  static constexpr bool kLocalMask = true;
   __device__ __forceinline__ int operator()(int l) const { return l; }
 };
 template <int LOGC, int LOGNS>  // 2^LOGC columns of a polynomial, column stride 2^LOGNS
 struct MapColumns {
   static constexpr bool kAligned = false;
+  static constexpr bool kLocalMask = false;
   int col0;
   __device__ __forceinline__ int operator()(int l) const {
     return col0 + (l & ((1 << LOGC) - 1)) + ((l >> LOGC) << LOGNS);
   }
 };
-struct MapBlock {  // contiguous block starting at base (a multiple of 8)
+struct MapBlock {  // contiguous block starting at base (a multiple of the local size)
   static constexpr bool kAligned = true;
+  static constexpr bool kLocalMask = true;
   int base;
   __device__ __forceinline__ int operator()(int l) const { return base + l; }
 };
 
-__device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, int e, uint64_t &w,
+__device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, unsigned e, uint64_t &w,
                                       uint64_t &wp) {
-  const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(tw + 2 * e);
+  // 32-bit byte offset from a (usually uniform) base: saddr addressing
+  const ulonglong2 v =
+      *reinterpret_cast<const ulonglong2 *>(reinterpret_cast<const char *>(tw) + (e << 4));
   w = v.x;
   wp = v.y;
 }
@@ -54,7 +59,7 @@ __device__ __forceinline__ cp_gi ld_gi(const CP_GLOBAL cp_gi *ptr) {
 __device__ __forceinline__ void st_gi(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
   *(CP_GLOBAL cp_u64x2 *)ptr = cp_u64x2{v.re, v.imag};
 }
-__device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw, int e,
+__device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw, unsigned e,
                                       uint64_t &w, uint64_t &wp) {
   const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)(tw + 2 * e);
   w = v.x;
@@ -193,11 +198,13 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
                                             uint64_t p, const Map &map, int ls0) {
   const int lst = LOGM - ls0 - LOG_EPT;
   const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+  const unsigned t0 = (unsigned)t & ((1u << lst) - 1);
 #pragma unroll
   for (int h = 0; h < LOG_EPT; h++) {
     const int half = EPT >> (h + 1);
     const int s = ls0 + h + SOFF;
     const int mmask = (1 << (LOGN - 1 - s)) - 1;
+    const unsigned e0 = t0 << s;
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       if (q & half) continue;
@@ -207,7 +214,9 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
       const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
       A.re = cp_red2(sr, p);
       A.imag = cp_red2(si, p);
-      const int e = (map(base + (q << lst)) & mmask) << s;
+      const unsigned e = Map::kLocalMask
+                             ? e0 + (unsigned)(q & (half - 1)) * (1u << (LOGM - LOG_EPT + SOFF + h))
+                             : (unsigned)((map(base + (q << lst)) & mmask) << s);
       uint64_t w, wp;
       ld_tw(tw, e, w, wp);
       B.re = cp_shoup(dr, w, wp, p);
@@ -223,17 +232,21 @@ __device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
                                             uint64_t p, const Map &map, int ls0) {
   const int lst = LOGM - ls0 - LOG_EPT;
   const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+  const unsigned t0 = (unsigned)t & ((1u << lst) - 1);
 #pragma unroll
   for (int h = LOG_EPT - 1; h >= 0; h--) {
     const int half = EPT >> (h + 1);
     const int s = ls0 + h + SOFF;
     const int mmask = (1 << (LOGN - 1 - s)) - 1;
+    const unsigned e0 = t0 << s;
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       if (q & half) continue;
       cp_gi &A = x[q];
       cp_gi &B = x[q + half];
-      const int e = (map(base + (q << lst)) & mmask) << s;
+      const unsigned e = Map::kLocalMask
+                             ? e0 + (unsigned)(q & (half - 1)) * (1u << (LOGM - LOG_EPT + SOFF + h))
+                             : (unsigned)((map(base + (q << lst)) & mmask) << s);
       uint64_t w, wp;
       ld_tw(tw, e, w, wp);
       const uint64_t tr = cp_shoup(B.re, w, wp, p);

@@ -168,9 +168,9 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
   e->t = t;
   for (int r = 0; r < e->nres; r++) {
     uint64_t p = cupoly_table_primes[r];
-    if (!(p > (1ULL << 62) && p < (1ULL << 63) && (1ULL << 63) - p < (1ULL << 32))) {
+    if (!(p > (1ULL << 62) && p < (1ULL << 63) && (1ULL << 63) - p < (1ULL << 30))) {
       delete e;
-      return cp_fail(CUPOLY_ERR_UNSUPPORTED, "prime outside (2^63 - 2^32, 2^63)");
+      return cp_fail(CUPOLY_ERR_UNSUPPORTED, "prime outside (2^63 - 2^30, 2^63)");
     }
     e->primes.push_back(p);
   }

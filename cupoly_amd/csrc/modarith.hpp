@@ -7,13 +7,12 @@
 //   * products by a constant known in advance (twiddles, twists, RNS
 //     constants) use Shoup's precomputed quotient w' = floor(w * 2^64 / p):
 //     one high and two low 64-bit products, valid for any 64-bit input;
-//   * products of two variables use Moller-Granlund division by the
-//     normalised divisor 2p with a precomputed inverse ("Improved division by
-//     invariant integers", IEEE TC 2011, Alg. 4), then one correction.
-// Both need p in (2^62, 2^63); the Shoup remainder also uses p = 2^63 - delta
-// with delta < 2^32.  Every prime of the 63-bit bucket meets both (the engine
-// checks).  On gfx950 the 128-bit products lower to
-// v_mad_u64_u32 / v_mul_hi_u32 chains on the integer pipes.
+//   * products of two variables fold the 127-bit product twice with
+//     2^63 = delta (mod p) (pseudo-Mersenne reduction), then one correction.
+// Both use p = 2^63 - delta with delta < 2^30; every prime of the 63-bit
+// bucket has delta < 2^29 (the engine checks the bound).  On gfx950 the
+// 128-bit products lower to v_mad_u64_u32 / v_mul_hi_u32 chains on the
+// integer pipes.
 #pragma once
 #include <stdint.h>
 
@@ -27,11 +26,9 @@ struct alignas(16) cp_gi {  // GaussianInteger {re, imag}, 16 bytes (include/cuP
   uint64_t re, imag;
 };
 
-// Per-prime constants for exact reduction.
+// One RNS prime, 2^63 - 2^30 < p < 2^63.
 struct cp_prime {
-  uint64_t p;     // the prime, 2^62 < p < 2^63
-  uint64_t d;     // 2p (normalised divisor, top bit set)
-  uint64_t dinv;  // floor((2^128 - 1) / d) - 2^64
+  uint64_t p;
 };
 
 CP_HD uint64_t cp_mulhi(uint64_t a, uint64_t b) {
@@ -47,15 +44,24 @@ CP_HD uint64_t cp_sub(uint64_t a, uint64_t b, uint64_t p) {
   return a >= b ? a - b : a + (p - b);
 }
 
-// Reduce from [0, 2p) to [0, p): the borrow of a - p selects (no separate compare).
+// Reduce from [0, 2p) to [0, p), p < 2^63.  t = a - p is formed as a + (2^64 - p):
+// for a >= p it is below p (top bit clear), for a < p it is at least
+// 2^64 - p > 2^63 (top bit set), so the sign of t selects.  On the GPU this is
+// one 64-bit add (v_lshl_add_u64), one compare and two selects; the empty asm
+// hides 2^64 - p from the optimizer, which would otherwise turn the add back
+// into a two-instruction subtract with borrow.
 CP_HD uint64_t cp_red2(uint64_t a, uint64_t p) {
-  uint64_t t;
-  return __builtin_sub_overflow(a, p, &t) ? a : t;
+  uint64_t np = 0 - p;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(np));
+#endif
+  const uint64_t t = a + np;
+  return (int64_t)t < 0 ? a : t;
 }
 
 // Shoup: a * w mod p in [0, 2p) for any a < 2^64, w < p, wp = floor(w 2^64 / p).
 // The remainder a w - q p is formed mod 2^64 as a w + q (2^64 - p); with
-// p = 2^63 - delta, delta < 2^32 (every prime of the 63-bit bucket), that is
+// p = 2^63 - delta, delta < 2^30 (every prime of the 63-bit bucket), that is
 // a w + q delta + (q mod 2) 2^63: two 32x32+64 multiply-adds for the low word
 // and three 32-bit products for the high word, and no 64-bit subtraction.
 CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
@@ -73,22 +79,26 @@ CP_HD uint64_t cp_shoup(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
   return cp_red2(cp_shoup_lazy(a, w, wp, p), p);
 }
 
-// (u1:u0) mod pr.d for u1 < pr.d, via the precomputed inverse.
-CP_HD uint64_t cp_rem2(uint64_t u1, uint64_t u0, const cp_prime& pr) {
-  cp_u128 q = (cp_u128)pr.dinv * u1;
-  q += ((cp_u128)(u1 + 1) << 64) | u0;
-  uint64_t q1 = (uint64_t)(q >> 64), q0 = (uint64_t)q;
-  uint64_t r = u0 - q1 * pr.d;
-  if (r > q0) r += pr.d;
-  if (r >= pr.d) r -= pr.d;
-  return r;
-}
-
-// a * b mod p for a < 2^64, b < 2^63 (so the high word of a*b is below 2p).
+// a * b mod p for a * b < 2^127 (a < 2^64, b < 2^63 or the reverse), with
+// p = 2^63 - delta, delta < 2^30 (every prime of the 63-bit bucket has
+// delta < 2^29).  Pseudo-Mersenne folding, since 2^63 = delta (mod p):
+//   x = H 2^63 + L           ->  y = L + H delta  < 2^94,
+//   y = H2 2^63 + L2         ->  z = L2 + H2 delta < 2^63 + 2^61 < 2p,
+// then one conditional subtraction.  Four 32x32 products for x, two for
+// H delta and one for H2 delta: about half the instructions of a division-
+// based reduction of the full product.
 CP_HD uint64_t cp_mulmod(uint64_t a, uint64_t b, const cp_prime& pr) {
-  cp_u128 x = (cp_u128)a * b;
-  uint64_t r = cp_rem2((uint64_t)(x >> 64), (uint64_t)x, pr);
-  return r >= pr.p ? r - pr.p : r;
+  const uint64_t p = pr.p;
+  const uint32_t delta = (uint32_t)((1ULL << 63) - p);
+  const cp_u128 x = (cp_u128)a * b;
+  const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  const uint64_t L = lo & ((1ULL << 63) - 1);
+  const uint64_t H = (hi << 1) | (lo >> 63);
+  const cp_u128 y = (cp_u128)H * delta + L;
+  const uint64_t L2 = (uint64_t)y & ((1ULL << 63) - 1);
+  const uint32_t H2 = (uint32_t)(y >> 63);
+  const uint64_t z = L2 + (uint64_t)H2 * delta;
+  return cp_red2(z, p);
 }
 
 // Gaussian integer product (a.re + i a.im)(b.re + i b.im) mod p, Karatsuba
@@ -124,9 +134,6 @@ static inline uint64_t cp_shoup_factor(uint64_t w, uint64_t p) {
 static inline cp_prime cp_make_prime(uint64_t p) {
   cp_prime pr;
   pr.p = p;
-  pr.d = p << 1;
-  cp_u128 all = ~(cp_u128)0;
-  pr.dinv = (uint64_t)(all / pr.d);  // floor((2^128-1)/d) - 2^64 (wraps the 2^64)
   return pr;
 }
 

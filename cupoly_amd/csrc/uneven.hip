@@ -74,14 +74,12 @@ struct cupoly_uneven {
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
-// Dynamic LDS of k_uneven (UNEVEN_SLOTS slots), named at file scope so that
-// the out-of-line tile bodies address it as LDS rather than through a generic
-// pointer.
+// Dynamic LDS of k_uneven (UNEVEN_SLOTS slots).
 extern __shared__ cp_gi uneven_lds[];
 
-// Generic -> global address space.  Pointers that arrive through memory or
-// as arguments of an out-of-line function are generic; typed this way the
-// loads and stores through them are global_* instead of flat_*.
+// Generic -> global address space.  Pointers read from memory (the work
+// list) are generic; typed this way the loads and stores through them are
+// global_* instead of flat_*.
 template <class T>
 __device__ __forceinline__ CP_GLOBAL T *glb(T *ptr) {
   return (CP_GLOBAL T *)ptr;
@@ -98,15 +96,15 @@ __device__ __forceinline__ cp_unit ld_unit(const cp_unit *units, int i) {
 }
 __device__ __forceinline__ cp_prime ld_prime(const cp_prime *primes, int i) {
   const CP_GLOBAL cp_prime *q = glb(primes) + i;
-  return cp_prime{q->p, q->d, q->dinv};
+  return cp_prime{q->p};
 }
 
-// The tile bodies are separate functions: the register allocator then gives
-// each its own spill frame and the kernel's scratch is the largest of them
-// instead of their sum.  Tables of the tile's ring degree come as separate
-// pointer arguments (twiddles, twists, primes).
+// Tile bodies, one per degree class, inlined into the persistent kernel
+// (measured faster than out-of-line calls, whose callee-saved registers cost
+// more scratch traffic than the larger shared spill frame).  Tables of the
+// tile's ring degree come as separate pointer arguments.
 template <int LOGN>
-__device__ __noinline__ void uneven_small(const cp_unit *units, int unit0, int nrows,
+__device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, int nrows,
                                           const uint64_t *twf_c, const uint64_t *twi_c,
                                           const uint64_t *tsf_c, const uint64_t *tsi_c,
                                           const cp_prime *primes) {
@@ -161,7 +159,7 @@ __device__ __noinline__ void uneven_small(const cp_unit *units, int unit0, int n
 }
 
 // n = 2^15: 16384 slots; scratch = 2 x 16384 slots private to this workgroup.
-__device__ __noinline__ void uneven_16k(const cp_unit *units, int unit0, const uint64_t *twf_c,
+__device__ __forceinline__ void uneven_16k(const cp_unit *units, int unit0, const uint64_t *twf_c,
                                         const uint64_t *twi_c, const uint64_t *tsf_c,
                                         const uint64_t *tsi_c, const cp_prime *primes,
                                         cp_gi *scratch) {

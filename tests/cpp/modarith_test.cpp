@@ -41,12 +41,20 @@ int main(int argc, char **argv) {
                     (unsigned long long)a, (unsigned long long)b);
         return 1;
       }
+      // canonical a times any 64-bit b (execute_mul_dgt_u64)
+      const uint64_t a3 = it < 144 ? edge[it % 12] % p : next64() % p;
+      const uint64_t b3 = it < 144 ? edge[(it / 12) % 12] : next64();
+      if (cp_mulmod(a3, b3, pr) != (uint64_t)(((cp_u128)a3 * b3) % p)) {
+        std::printf("mulmod (b < 2^64) FAIL p=%llu a=%llu b=%llu\n", (unsigned long long)p,
+                    (unsigned long long)a3, (unsigned long long)b3);
+        return 1;
+      }
       const uint64_t r = next64() % (2 * p);
       if (cp_red2(r, p) != r % p) {
         std::printf("red2 FAIL\n");
         return 1;
       }
-      checks += 3;
+      checks += 4;
     }
   }
   std::printf("modarith ok (%ld checks)\n", checks);
