@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   // table pointer) wave-uniform so it lives in SGPRs
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
-  cp_gi *lds = lds_all + row * N;
+  cp_gi *lds = lds_all + row * lds_row<LOGN>();
   int unit = blockIdx.x * ROWS + row;
   const bool active = unit < nunits;
   if (!active) unit = nunits - 1;
@@ -64,18 +64,18 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
     dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
     using LP = LastPass<LOGN, 0, LOGN>;
     __syncthreads();
-    lds_store<LOGN, LP::LS0, LP::RL>(lds, x, t);
+    lds_store<LOGN, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     __syncthreads();
     if (active) {
 #pragma unroll
-      for (int i = 0; i < EPT; i++) poly[nat_index<LOGN>(t, i)] = lds[nat_index<LOGN>(t, i)];
+      for (int i = 0; i < EPT; i++) poly[nat_index<LOGN>(t, i)] = lds[lds_phys(nat_index<LOGN>(t, i), LP::MS)];
     }
   } else {
     using LP = LastPass<LOGN, 0, LOGN>;
 #pragma unroll
-    for (int i = 0; i < EPT; i++) lds[nat_index<LOGN>(t, i)] = poly[nat_index<LOGN>(t, i)];
+    for (int i = 0; i < EPT; i++) lds[lds_phys(nat_index<LOGN>(t, i), LP::MS)] = poly[nat_index<LOGN>(t, i)];
     __syncthreads();
-    lds_load<LOGN, LP::LS0, LP::RL>(lds, x, t);
+    lds_load<LOGN, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     dit_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
     if (active) {
 #pragma unroll
@@ -134,16 +134,16 @@ __global__ void __launch_bounds__(1024)
     dif_passes<LOGN, LOGM, 0, SP::S, 0>(x, lds, t, tw, p, map);
     using LP = LastPass<LOGM, 0, SP::S>;
     __syncthreads();
-    lds_store<LOGM, LP::LS0, LP::RL>(lds, x, t);
+    lds_store<LOGM, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGM>(t, i))] = lds[nat_index<LOGM>(t, i)];
+    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGM>(t, i))] = lds[lds_phys(nat_index<LOGM>(t, i), LP::MS)];
   } else {
     using LP = LastPass<LOGM, 0, SP::S>;
 #pragma unroll
-    for (int i = 0; i < EPT; i++) lds[nat_index<LOGM>(t, i)] = poly[map(nat_index<LOGM>(t, i))];
+    for (int i = 0; i < EPT; i++) lds[lds_phys(nat_index<LOGM>(t, i), LP::MS)] = poly[map(nat_index<LOGM>(t, i))];
     __syncthreads();
-    lds_load<LOGM, LP::LS0, LP::RL>(lds, x, t);
+    lds_load<LOGM, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     dit_passes<LOGN, LOGM, 0, SP::S, 0>(x, lds, t, tw, p, map);
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(1024)
   const int rid = roff + unit % nres;
   const uint64_t p = primes[rid].p;
   const uint64_t *tw = tw_all + (size_t)rid * N;
-  cp_gi *lds = lds_all + row * B;
+  cp_gi *lds = lds_all + row * lds_row<LOGB>();
   cp_gi *poly = data + (size_t)unit * N;
   MapBlock map{blk << LOGB};
   cp_gi x[EPT];
@@ -180,15 +180,15 @@ __global__ void __launch_bounds__(1024)
     for (int i = 0; i < EPT; i++) x[i] = poly[map(first_index<LOGB, 0, LOGB>(t, i))];
     dif_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
     __syncthreads();
-    lds_store<LOGB, LP::LS0, LP::RL>(lds, x, t);
+    lds_store<LOGB, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGB>(t, i))] = lds[nat_index<LOGB>(t, i)];
+    for (int i = 0; i < EPT; i++) poly[map(nat_index<LOGB>(t, i))] = lds[lds_phys(nat_index<LOGB>(t, i), LP::MS)];
   } else {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) lds[nat_index<LOGB>(t, i)] = poly[map(nat_index<LOGB>(t, i))];
+    for (int i = 0; i < EPT; i++) lds[lds_phys(nat_index<LOGB>(t, i), LP::MS)] = poly[map(nat_index<LOGB>(t, i))];
     __syncthreads();
-    lds_load<LOGB, LP::LS0, LP::RL>(lds, x, t);
+    lds_load<LOGB, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
     dit_passes<LOGN, LOGB, 0, LOGB, SP::S>(x, lds, t, tw, p, map);
 #pragma unroll
     for (int i = 0; i < EPT; i++) poly[map(first_index<LOGB, 0, LOGB>(t, i))] = x[i];
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   // table pointer) wave-uniform so it lives in SGPRs
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
-  cp_gi *lds = lds_all + row * N;
+  cp_gi *lds = lds_all + row * lds_row<LOGN>();
   int unit = blockIdx.x * ROWS + row;
   const bool active = unit < nunits;
   if (!active) unit = nunits - 1;
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(1024)
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
   const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
-  cp_gi *lds = lds_all + row * B;
+  cp_gi *lds = lds_all + row * lds_row<LOGB>();
   cp_gi *pa = a + (size_t)unit * N;
   const cp_gi *pb = b + (size_t)unit * N;
   MapBlock map{blk << LOGB};
@@ -320,12 +320,12 @@ static int setup_size() {
   int rc;
   if constexpr (LOGN <= 13) {
     constexpr int R = rows_for<LOGN>();
-    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
     if ((rc = set_lds(k_dgt_single<LOGN, R, true>, lds))) return rc;
     if ((rc = set_lds(k_dgt_single<LOGN, R, false>, lds))) return rc;
     if ((rc = set_lds(k_polymul_fused<LOGN, R>, lds))) return rc;
   } else {
-    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     if ((rc = set_lds(k_dgt_cols<LOGN, true>, lds))) return rc;
     if ((rc = set_lds(k_dgt_cols<LOGN, false>, lds))) return rc;
     if ((rc = set_lds(k_dgt_blocks<LOGN, true>, lds))) return rc;
@@ -360,7 +360,7 @@ static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npol
   if (nunits <= 0) return CUPOLY_OK;
   if constexpr (LOGN <= 13) {
     constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
-    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
     const int grid = (nunits + R - 1) / R;
     if (dir == CUPOLY_FORWARD)
       hipLaunchKernelGGL((k_dgt_single<LOGN, R, true>), dim3(grid), dim3(T * R), lds, c->stream,
@@ -370,7 +370,7 @@ static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npol
                          data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
   } else {
     using SP = Split<LOGN>;
-    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     const int gcols = nunits * SP::WG_PER_POLY;
     const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
     if (dir == CUPOLY_FORWARD) {
@@ -414,7 +414,7 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
   if (nunits <= 0) return CUPOLY_OK;
   if constexpr (LOGN <= 13) {
     constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
-    const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+    const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
     hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
                        c->stream, out, a, b, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
                        e->d_twist_fwd, e->d_twist_inv);
@@ -423,7 +423,7 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     // out <- a, scratch <- b (caller inputs are not modified); transforms in place
     using SP = Split<LOGN>;
     const size_t bytes = (size_t)nunits * (1 << LOGN) * sizeof(cp_gi);
-    const size_t lds = (size_t)8192 * sizeof(cp_gi);
+    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     if (out != a) CP_HIP(hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream));
     cp_gi *tb = nullptr;
     CP_HIP(hipMallocAsync((void **)&tb, bytes, c->stream));

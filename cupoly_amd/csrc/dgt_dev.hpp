@@ -170,25 +170,72 @@ __device__ __forceinline__ int pass_index_rt(int t, int i, int ls0) {
   return ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1)) + (i << lst);
 }
 
-template <int LOGM, int LS0, int R>
+// ---- LDS layout of an exchange ---------------------------------------------
+// An exchange writes the registers in one pass layout and reads them in the
+// next.  A 16-lane group of 16-byte accesses is conflict-free when it hits 16
+// distinct slots modulo 256 bytes; layouts with unit stride ST = 1 (8
+// consecutive slots per lane) or ST = 8 (runs of 8 lanes 64 slots apart) hit
+// only 2 of them.  Each exchange therefore pads the array linearly,
+//     physical = l + A * (l >> S),
+// with (A, S) = (1, 3) when the smaller stride of its two layouts is 1,
+// (8, 6) when it is 8, and no padding otherwise.  Both maps are
+// conflict-free for both layouts of their exchange (and for the natural
+// order used to stage global I/O), and keep every address a per-thread base
+// plus i times a uniform stride.  A 2^LOGM-slot array then spans
+// 2^LOGM + 2^LOGM / 8 slots.
+struct LdsMap {
+  int A, S;
+};
+__host__ __device__ constexpr LdsMap lds_map_for(int lst_lo) {
+  return lst_lo == 0 ? LdsMap{1, 3} : lst_lo == 3 ? LdsMap{8, 6} : LdsMap{0, 0};
+}
+template <int LOGM>
+__host__ __device__ constexpr int lds_row() {
+  return (1 << LOGM) + ((1 << LOGM) >> 3);
+}
+__device__ __forceinline__ int lds_phys(int l, LdsMap m) { return l + m.A * (l >> m.S); }
+
+template <int LOGM, int LS0, int R, int A, int S>
 __device__ __forceinline__ void lds_store(cp_gi *lds, const cp_gi (&x)[EPT], int t) {
 #pragma unroll
-  for (int i = 0; i < EPT; i++) lds[pass_index<LOGM, LS0, R>(t, i)] = x[i];
+  for (int i = 0; i < EPT; i++) {
+    const int l = pass_index<LOGM, LS0, R>(t, i);
+    lds[l + A * (l >> S)] = x[i];
+  }
 }
-template <int LOGM, int LS0, int R>
+template <int LOGM, int LS0, int R, int A, int S>
 __device__ __forceinline__ void lds_load(const cp_gi *lds, cp_gi (&x)[EPT], int t) {
 #pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index<LOGM, LS0, R>(t, i)];
+  for (int i = 0; i < EPT; i++) {
+    const int l = pass_index<LOGM, LS0, R>(t, i);
+    x[i] = lds[l + A * (l >> S)];
+  }
+}
+// Full radix-8 layout at runtime stage ls0 (stride 2^lst) under map m, which
+// must have S <= lst or S == lst + 3 (true for the maps chosen above):
+// physical(base + i 2^lst) = PB + i D.
+template <int LOGM>
+__device__ __forceinline__ void lds_rt_addr(int t, int ls0, LdsMap m, int &pb, int &d) {
+  const int lst = LOGM - ls0 - LOG_EPT;
+  const int b = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+  pb = b + m.A * (b >> m.S);
+  d = (1 << lst) + (lst >= m.S ? m.A << (lst - m.S) : 0);
 }
 template <int LOGM>
-__device__ __forceinline__ void lds_store_rt(cp_gi *lds, const cp_gi (&x)[EPT], int t, int ls0) {
+__device__ __forceinline__ void lds_store_rt(cp_gi *lds, const cp_gi (&x)[EPT], int t, int ls0,
+                                             LdsMap m) {
+  int pb, d;
+  lds_rt_addr<LOGM>(t, ls0, m, pb, d);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) lds[pass_index_rt<LOGM>(t, i, ls0)] = x[i];
+  for (int i = 0; i < EPT; i++) lds[pb + i * d] = x[i];
 }
 template <int LOGM>
-__device__ __forceinline__ void lds_load_rt(const cp_gi *lds, cp_gi (&x)[EPT], int t, int ls0) {
+__device__ __forceinline__ void lds_load_rt(const cp_gi *lds, cp_gi (&x)[EPT], int t, int ls0,
+                                            LdsMap m) {
+  int pb, d;
+  lds_rt_addr<LOGM>(t, ls0, m, pb, d);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = lds[pass_index_rt<LOGM>(t, i, ls0)];
+  for (int i = 0; i < EPT; i++) x[i] = lds[pb + i * d];
 }
 
 // Full radix-8 pass at a runtime local stage ls0 (never the last global
@@ -293,22 +340,24 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            TW __restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
+  constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);  // partial <-> first full
   if constexpr (PP::R0 > 0) {
     dif_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
     if constexpr (PP::NF == 0) return;
     __syncthreads();
-    lds_store<LOGM, LSB, PP::R0>(lds, x, t);
+    lds_store<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
     __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, PP::LSF);
+    lds_load_rt<LOGM>(lds, x, t, PP::LSF, MP);
   }
 #pragma nounroll
   for (int i = 0; i < PP::NF - 1; i++) {
     const int ls0 = PP::LSF + LOG_EPT * i;
+    const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
     dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
     __syncthreads();
-    lds_store_rt<LOGM>(lds, x, t, ls0);
+    lds_store_rt<LOGM>(lds, x, t, ls0, m);
     __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
+    lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
   }
   if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 }
@@ -320,22 +369,24 @@ __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            TW __restrict__ tw, uint64_t p,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
+  constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 #pragma nounroll
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
+    const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
     __syncthreads();
-    lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT);
+    lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     __syncthreads();
-    lds_load_rt<LOGM>(lds, x, t, ls0);
+    lds_load_rt<LOGM>(lds, x, t, ls0, m);
     dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
   }
   if constexpr (PP::R0 > 0) {
     if constexpr (PP::NF > 0) {
       __syncthreads();
-      lds_store_rt<LOGM>(lds, x, t, PP::LSF);
+      lds_store_rt<LOGM>(lds, x, t, PP::LSF, MP);
       __syncthreads();
-      lds_load<LOGM, LSB, PP::R0>(lds, x, t);
+      lds_load<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
     }
     dit_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
   }
@@ -346,6 +397,8 @@ template <int LOGM, int LSB, int LSE>
 struct LastPass {
   static constexpr int LS0 = PassPlan<LSB, LSE>::LSL;
   static constexpr int RL = PassPlan<LSB, LSE>::RL;
+  // map of an exchange between this layout and the natural order
+  static constexpr LdsMap MS = lds_map_for(LOGM - LS0 - RL);
 };
 
 // natural layout index of register i: t + i*T (LDS-staged global I/O)

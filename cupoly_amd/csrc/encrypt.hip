@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
-  cp_gi *lds = lds_all + row * N;
+  cp_gi *lds = lds_all + row * lds_row<LOGN>();
   int unit = blockIdx.x * ROWS + row;
   const bool active = unit < nunits;
   if (!active) unit = nunits - 1;
@@ -83,7 +83,7 @@ static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch,
   cupoly_engine *e = c->e;
   const int nres = e->k, nunits = nres * batch;
   constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
-  const size_t lds = (size_t)R * (1 << LOGN) * sizeof(cp_gi);
+  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   static bool attr_set = false;
   if (!attr_set && lds > 64 * 1024) {
     CP_HIP(hipFuncSetAttribute((const void *)k_sample_muladd<LOGN, R>,

@@ -23,7 +23,8 @@
 #include "dgt_dev.hpp"
 
 #define UNEVEN_THREADS 1024
-#define UNEVEN_SLOTS 8192  // Gaussian slots per tile (128 KiB of LDS)
+#define UNEVEN_SLOTS 8192  // Gaussian slots per tile
+#define UNEVEN_LDS_SLOTS 9216  // its padded LDS array (144 KiB)
 #define UNEVEN_NCLASS 5    // log2(n/2) = 10 .. 14
 
 struct cp_unit {
@@ -74,7 +75,7 @@ struct cupoly_uneven {
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
-// Dynamic LDS of k_uneven (UNEVEN_SLOTS slots).
+// Dynamic LDS of k_uneven (UNEVEN_LDS_SLOTS slots).
 extern __shared__ cp_gi uneven_lds[];
 
 // Generic -> global address space.  Pointers read from memory (the work
@@ -111,7 +112,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   constexpr int N = 1 << LOGN, TPR = N / EPT, R = UNEVEN_SLOTS / N;
   static_assert(R * TPR == UNEVEN_THREADS, "tile shape");
   const int row = (int)threadIdx.x / TPR, t = (int)threadIdx.x % TPR;
-  cp_gi *lds = uneven_lds + row * N;
+  cp_gi *lds = uneven_lds + row * lds_row<LOGN>();
   const bool active = row < nrows;
   const cp_unit u = ld_unit(units, unit0 + (active ? row : 0));
   const cp_prime pr = ld_prime(primes, u.rid);
@@ -299,7 +300,7 @@ static int ensure_scratch(cupoly_uneven *u, int grid) {
 static int launch_uneven(cupoly_uneven *u, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, u->d_counter);
   hipLaunchKernelGGL(k_uneven, dim3(u->grid), dim3(UNEVEN_THREADS),
-                     UNEVEN_SLOTS * sizeof(cp_gi), s, u->d_tiles, u->d_units, u->ntiles,
+                     UNEVEN_LDS_SLOTS * sizeof(cp_gi), s, u->d_tiles, u->d_units, u->ntiles,
                      u->d_counter, u->tabs, u->d_scratch);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
@@ -411,7 +412,7 @@ extern "C" int cupoly_uneven_create(cupoly_ctx *c, cupoly_engine *const *engines
   static bool attr = false;
   if (!attr) {
     err = hipFuncSetAttribute((const void *)k_uneven, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              UNEVEN_SLOTS * sizeof(cp_gi));
+                              UNEVEN_LDS_SLOTS * sizeof(cp_gi));
     if (err != hipSuccess) {
       cupoly_uneven_destroy(u);
       return cp_check_hip(err, "uneven_create: LDS attribute");
