@@ -68,13 +68,22 @@ __global__ void __launch_bounds__(256)
   hps_b_to_q_slot(xr, xi, out, ng, c, H);
 }
 
-// --- simple scaling Q -> t for t a power of two (cudaengine.cu:323-363).
+// --- simple scaling Q -> t (cudaengine.cu:323-363): round(t x / Q) mod t per
+// coefficient.  For each residue j, t (Q/q_j)^-1 mod q_j / q_j = w_int + w_frac
+// (w_frac as a 128-bit fraction): x_j w_int + floor(x_j w_frac) is summed mod t
+// and the fractional words are accumulated in FP64 and rounded once.  The
+// reference reduces mod t with masks, which is exact only for t a power of two
+// (for other t the product x_j w_int overflows, known-issues.md item 1); here
+// x_j is reduced mod t first, so any t < 2^32 is exact, and for powers of two
+// the result equals the masked one.
 __global__ void __launch_bounds__(256)
     k_simple_scaling(cp_gi *__restrict__ a, int ng, const cp_hps *__restrict__ H) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int k = H->k;
-  const uint64_t mask = H->t - 1;
+  const uint64_t t = H->t;
+  const bool pow2 = (t & (t - 1)) == 0;
+  const uint64_t mask = t - 1;
   const double two_m64 = 0x1p-64;
   double acc_r = 0.0, acc_i = 0.0;
   uint64_t rr = 0, ri = 0;
@@ -86,12 +95,22 @@ __global__ void __launch_bounds__(256)
     umul128hi(H->twfrac[j][0], H->twfrac[j][1], v.imag, li, hi);
     acc_r = fma((double)lr, two_m64, acc_r);
     acc_i = fma((double)li, two_m64, acc_i);
-    rr += ((v.re * tw & mask) + (hr & mask)) & mask;
-    ri += ((v.imag * tw & mask) + (hi & mask)) & mask;
+    if (pow2) {
+      rr += ((v.re * tw & mask) + (hr & mask)) & mask;
+      ri += ((v.imag * tw & mask) + (hi & mask)) & mask;
+    } else {  // t < 2^32: (x mod t) w_int < 2^64
+      rr += ((v.re % t) * tw % t + hr % t) % t;
+      ri += ((v.imag % t) * tw % t + hi % t) % t;
+    }
   }
   cp_gi o;
-  o.re = (rr + (uint64_t)llrint(acc_r)) & mask;
-  o.imag = (ri + (uint64_t)llrint(acc_i)) & mask;
+  if (pow2) {
+    o.re = (rr + (uint64_t)llrint(acc_r)) & mask;
+    o.imag = (ri + (uint64_t)llrint(acc_i)) & mask;
+  } else {
+    o.re = (rr + (uint64_t)llrint(acc_r)) % t;
+    o.imag = (ri + (uint64_t)llrint(acc_i)) % t;
+  }
   a[c] = o;
 }
 
@@ -124,9 +143,9 @@ extern "C" int cupoly_basis_ext_b_to_q(cupoly_ctx *c, cupoly_gi *b, const uint64
 
 extern "C" int cupoly_simple_scaling(cupoly_ctx *c, cupoly_gi *a) {
   if (!c || !a) return cp_fail(CUPOLY_ERR_ARG, "simple_scaling: null argument");
-  if (c->e->t & (c->e->t - 1))
-    return cp_fail(CUPOLY_ERR_UNSUPPORTED, "simple_scaling: t must be a power of 2 "
-                                           "(cudaengine.cu:339-357)");
+  if ((c->e->t & (c->e->t - 1)) && c->e->t >= (1ULL << 32))
+    return cp_fail(CUPOLY_ERR_UNSUPPORTED,
+                   "simple_scaling: t must be a power of 2 or below 2^32");
   hipLaunchKernelGGL(k_simple_scaling, slots_grid(c->e->ng), dim3(256), 0, c->stream, (cp_gi *)a,
                      c->e->ng, c->e->d_hps);
   CP_LAUNCH_CHECK();

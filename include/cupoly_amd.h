@@ -114,7 +114,9 @@ int cupoly_basis_ext_q_to_b(cupoly_ctx *c, cupoly_gi *a);
 int cupoly_complex_scaling(cupoly_ctx *c, uint64_t *b, const cupoly_gi *a_q, const cupoly_gi *a_b);
 /* execute_polynomial_basis_ext_B_to_Q(b, a, ctx): a = uint64[kl][n] -> b Q. */
 int cupoly_basis_ext_b_to_q(cupoly_ctx *c, cupoly_gi *b, const uint64_t *a);
-/* execute_polynomial_simple_scaling(a, ctx): Q -> T in place (t a power of 2). */
+/* execute_polynomial_simple_scaling(a, ctx): Q -> T in place, round(t x / Q) mod t.
+ * t a power of two (the reference's masked sums) or any t < 2^32 (exact
+ * reductions mod t; the reference rejects such t, known-issues.md item 1). */
 int cupoly_simple_scaling(cupoly_ctx *c, cupoly_gi *a);
 
 /* ---- poly_t-level compositions (polynomial.cu) ----------------------------

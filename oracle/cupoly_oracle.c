@@ -415,9 +415,13 @@ void or_basis_ext_b_to_q(uint64_t *out, const uint64_t *in, int Ng, int k, int k
 /* in: [k][Ng] GI coefficient domain; out: [Ng] GI residues mod t.           */
 /* twint[j] = floor(t * (Q/q_j)^-1 / q_j), twfrac[2j..] its fraction * 2^128 */
 /* ------------------------------------------------------------------------ */
+/* cudaengine.cu:323-363 for t a power of two (masks); for other t < 2^32
+ * the same sums with exact reductions mod t (x_j reduced before the
+ * product, which the reference's masked form cannot do). */
 void or_simple_scaling(uint64_t *out, const uint64_t *in, int Ng, int k, uint64_t t,
                        const uint64_t *twint, const uint64_t *twfrac) {
   const double two_m64 = 0x1p-64;
+  const int pow2 = (t & (t - 1)) == 0;
   uint64_t mask = t - 1;
   for (int c = 0; c < Ng; c++) {
     double acc[2] = {0, 0};
@@ -425,14 +429,18 @@ void or_simple_scaling(uint64_t *out, const uint64_t *in, int Ng, int k, uint64_
     for (int j = 0; j < k; j++)
       for (int h = 0; h < 2; h++) {
         uint64_t v = in[2 * ((size_t)j * Ng + c) + h];
-        uint64_t vint = (v * twint[j]) & mask;
         uint64_t lo, hi;
         umul128hi(twfrac[2 * j], twfrac[2 * j + 1], v, &lo, &hi);
         acc[h] = fma((double)lo, two_m64, acc[h]);
-        res[h] += (vint + (hi & mask)) & mask;
+        if (pow2)
+          res[h] += (((v * twint[j]) & mask) + (hi & mask)) & mask;
+        else
+          res[h] += ((v % t) * twint[j] % t + hi % t) % t;
       }
-    for (int h = 0; h < 2; h++)
-      out[2 * (size_t)c + h] = (res[h] + (uint64_t)llrint(acc[h])) & mask;
+    for (int h = 0; h < 2; h++) {
+      uint64_t r = res[h] + (uint64_t)llrint(acc[h]);
+      out[2 * (size_t)c + h] = pow2 ? (r & mask) : (r % t);
+    }
   }
 }
 

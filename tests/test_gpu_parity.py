@@ -147,10 +147,28 @@ def test_hps_kernels_match_oracle(gpu, k, n, t):
     oq = cp.empty((k, P.Ng, 2))
     cp.execute_polynomial_basis_ext_B_to_Q(ctx, oq, out)
     assert np.array_equal(cp.to_host(oq), O.basis_ext_b_to_q(P, cs))
-    if t & (t - 1) == 0:
-        d = cp.to_device(xq)
-        cp.execute_polynomial_simple_scaling(ctx, d)
-        assert np.array_equal(cp.to_host(d)[0], O.simple_scaling(P, xq))
+    d = cp.to_device(xq)
+    cp.execute_polynomial_simple_scaling(ctx, d)
+    assert np.array_equal(cp.to_host(d)[0], O.simple_scaling(P, xq))
+
+
+@pytest.mark.parametrize("k,n,t", [(4, 16384, 65537), (3, 4096, 256), (2, 2048, 40961)])
+def test_simple_scaling_recovers_scaled_messages(gpu, k, n, t):
+    """Decryption shape: x = floor(Q/t) m + e (|e| small) scaled by t/Q and
+    rounded gives m back, for t = 65537 (not a power of two) as for 256."""
+    e_, ctx = engine(k, k + 1, n, t)
+    P = O.Params(k, k + 1, n, t)
+    rng = np.random.default_rng(t + n)
+    m = rng.integers(0, t, size=n)
+    err = rng.integers(-1000, 1001, size=n)
+    delta = P.Q // t
+    x = [(delta * int(mi) + int(ei)) % P.Q for mi, ei in zip(m, err)]
+    res = O.fold(np.array([[v % q for v in x] for q in P.q], dtype=np.uint64), P.Ng)
+    d = cp.to_device(res)
+    cp.execute_polynomial_simple_scaling(ctx, d)
+    got = O.unfold(cp.to_host(d)[0])
+    assert np.array_equal(got, m.astype(np.uint64))
+    assert np.array_equal(cp.to_host(d)[0], O.simple_scaling(P, res))
 
 
 def test_hps_poly_level_compositions(gpu):
