@@ -24,7 +24,8 @@ LIB_PATH = os.path.join(HERE, "libcupoly_amd.so")
 QBase, BBase, QBBase, TBase = 0, 1, 2, 3
 FORWARD, INVERSE = 0, 1
 ADD, SUB, MUL, MULMUL, ADDADD = 0, 1, 2, 3, 4
-DISCRETE_GAUSSIAN, UNIFORM, TERNARY = 0, 5, 6
+DISCRETE_GAUSSIAN, HAMMINGWEIGHT, NARROW, ZO, BINARY, UNIFORM, TERNARY = 0, 1, 2, 3, 4, 5, 6
+HAMMINGWEIGHT_H = 64
 
 _u64 = ctypes.c_uint64
 _int = ctypes.c_int
@@ -69,6 +70,9 @@ SIGNATURES = {
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
+    "cupoly_xi_bfv": ([_vp, _vp, _vp], _int),
+    "cupoly_rho_bfv": ([_vp, _vp, _vp, _int], _int),
+    "cupoly_dot": ([_vp, _vp, _vp, _vp, _int, _int], _int),
     "cupoly_uneven_create": ([_vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _int,
                               ctypes.POINTER(_vp)], _int),
     "cupoly_uneven_run": ([_vp], _int),
@@ -370,6 +374,44 @@ def sample_muladd_batch(ctx, out, a, seed, stream0):
     _need(a, e.k, e.N, batch)
     _need(out, e.k, e.N, batch)
     _check(lib().cupoly_sample_muladd_batch(ctx.handle, _ptr(out), _ptr(a), batch, seed, stream0))
+
+
+def _ptr_array(ts):
+    return (ctypes.c_void_p * max(len(ts), 1))(*[_ptr(t).value for t in ts])
+
+
+def execute_xi_bfv(ctx, outs, a):
+    """poly_xi_bfv: residue i of outs[i] = a_i (Q/q_i)^-1 mod q_i (base Q)."""
+    e = ctx.engine
+    _need(a, e.k, e.N)
+    if len(outs) != e.k:
+        raise ValueError("cupoly_amd: xi needs k = %d outputs" % e.k)
+    for o in outs:
+        _need(o, e.k, e.N)
+    _check(lib().cupoly_xi_bfv(ctx.handle, _ptr_array(outs), _ptr(a)))
+
+
+def execute_rho_bfv(ctx, outs, a, base=QBase):
+    """poly_rho_bfv: residue i of outs[i] = a_i (Q/q_i) mod q_i, i < k."""
+    e = ctx.engine
+    nres = e.get_n_residues(base)
+    _need(a, nres, e.N)
+    if len(outs) != e.k:
+        raise ValueError("cupoly_amd: rho needs k = %d outputs" % e.k)
+    for o in outs:
+        _need(o, nres, e.N)
+    _check(lib().cupoly_rho_bfv(ctx.handle, _ptr_array(outs), _ptr(a), base))
+
+
+def dot(ctx, out, a, b, base=QBase):
+    """poly_dot: out = sum_i a[i] * b[i] in the DGT domain."""
+    e = ctx.engine
+    nres = e.get_n_residues(base)
+    if len(a) != len(b) or not a:
+        raise ValueError("cupoly_amd: dot needs two equal, non-empty lists")
+    for t in list(a) + list(b) + [out]:
+        _need(t, nres, e.N)
+    _check(lib().cupoly_dot(ctx.handle, _ptr(out), _ptr_array(a), _ptr_array(b), len(a), base))
 
 
 class UnevenPlan:

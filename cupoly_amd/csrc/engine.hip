@@ -101,6 +101,7 @@ static void compute_hps(cp_hps &h, const uint64_t *primes, int k, int kl, uint64
   for (int j = 0; j < k; j++) {
     uint64_t Qj = prodQ_except(j, q[j]);  // (Q/q_j) mod q_j
     shoup_pair(invm(Qj, q[j]), q[j], h.invQqi[j]);
+    shoup_pair(Qj, q[j], h.Qqi[j]);
     shoup_pair(prodB_except(-1, q[j]), q[j], h.Bqi[j]);  // B mod q_j
     h.rcp_q[j] = rcp_rd_exact(q[j]);
     for (int i = 0; i < kl; i++) {
@@ -284,6 +285,7 @@ static int hps_list(const cp_hps &h, const char *name, uint64_t *out, int cap) {
   std::string s(name);
   const int k = h.k, kl = h.kl;
   if (s == "invQqi") for (int j = 0; j < k; j++) v.push_back(h.invQqi[j][0]);
+  else if (s == "Qqi") for (int j = 0; j < k; j++) v.push_back(h.Qqi[j][0]);
   else if (s == "Qqibi") for (int i = 0; i < kl; i++) for (int j = 0; j < k; j++) v.push_back(h.Qqibi[i][j][0]);
   else if (s == "Qbi") for (int i = 0; i < kl; i++) v.push_back(h.Qbi[i][0]);
   else if (s == "invBbi") for (int i = 0; i < kl; i++) v.push_back(h.invBbi[i][0]);

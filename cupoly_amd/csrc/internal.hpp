@@ -18,6 +18,7 @@ struct cp_hps {
   uint64_t t;
   cp_prime q[CP_MAX_BASE], b[CP_MAX_BASE];
   uint64_t invQqi[CP_MAX_BASE][2];               // (Q/q_j)^-1 mod q_j
+  uint64_t Qqi[CP_MAX_BASE][2];                  // (Q/q_j) mod q_j
   uint64_t Qqibi[CP_MAX_BASE][CP_MAX_BASE][2];   // [i][j] (Q/q_j) mod b_i
   uint64_t Qbi[CP_MAX_BASE][2];                  // Q mod b_i
   double rcp_q[CP_MAX_BASE];                     // RD(1 / RN(q_j))

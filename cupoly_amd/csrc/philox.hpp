@@ -80,6 +80,32 @@ __device__ __forceinline__ int gaussian_coef(uint64_t seed, uint64_t stream, uin
   return idx - CUPOLY_GAUSS_BOUND;
 }
 
+__device__ __forceinline__ int narrow_coef(uint64_t seed, uint64_t stream, uint32_t coef) {
+  for (uint32_t att = 0;; att++) {
+    uint32_t w[4];
+    draw(seed, stream, coef, 4, 0, att, w);
+#pragma unroll
+    for (int h = 0; h < 4; h++)
+      if (w[h] < 4294967295u) return (int)(w[h] % 3u);
+  }
+}
+
+__device__ __forceinline__ int binary_coef(uint64_t seed, uint64_t stream, uint32_t coef) {
+  uint32_t w[4];
+  draw(seed, stream, coef, 5, 0, 0, w);
+  return (int)(w[0] & 1u);
+}
+
+// Fixed-weight sampling key of coefficient coef (< 2^16): 48 random bits
+// above the index, so keys are distinct; neg = sign of a selected coefficient.
+__device__ __forceinline__ uint64_t fixed_weight_key(uint64_t seed, uint64_t stream, uint32_t coef,
+                                                     bool &neg) {
+  uint32_t w[4];
+  draw(seed, stream, coef, 6, 0, 0, w);
+  neg = (w[2] & 1u) != 0;
+  return ((((uint64_t)w[0] << 16) | (w[1] >> 16)) << 16) | (coef & 0xFFFFu);
+}
+
 // small signed value -> residue mod q
 __device__ __forceinline__ uint64_t small_mod(int e, uint64_t q) {
   return e < 0 ? q - (uint64_t)(-e) : (uint64_t)e;

@@ -12,7 +12,20 @@
 //   ternary {-1,0,1} (tag 2): 32-bit words, rejection of 2^32 - 1, w mod 3 - 1;
 //   Gaussian         (tag 3): sigma 3.2 truncated at |e| <= 19 by inversion of
 //                              a 64-bit cumulative table (generated with
-//                              60-digit arithmetic).
+//                              60-digit arithmetic);
+//   narrow {0,1,2}   (tag 4): as ternary, value w mod 3 (the reference's BFV-
+//                              mode narrow sampler, sampler.cu:72-103);
+//   binary {0,1}     (tag 5): bit 0 of the first word (sampler.cu:158-182);
+//   ZO / Hamming weight (tag 6): exactly h coefficients +-1 (h = n/2 for ZO,
+//                              h = 64 for HAMMINGWEIGHT, settings.h:43), at the
+//                              positions of the h smallest keys (48 random
+//                              bits above the 16-bit index, so all distinct),
+//                              sign from bit 0 of the third word.  The
+//                              reference draws these on the host with rand()
+//                              and std::shuffle (sampler.cu:222-278, 401-446);
+//                              here one workgroup radix-selects the h-th
+//                              smallest key in eight 8-bit passes over LDS
+//                              histograms.
 // Small values are written to every residue of the base as e mod q_i, like the
 // reference's samplers (sampler.cu:94-97, 485-487).
 #include "philox.hpp"
@@ -45,6 +58,12 @@ __global__ void __launch_bounds__(256)
   if constexpr (KIND == CUPOLY_TERNARY) {
     e0 = ternary_coef(seed, stream, (uint32_t)slot);
     e1 = ternary_coef(seed, stream, (uint32_t)(slot + ng));
+  } else if constexpr (KIND == CUPOLY_NARROW) {
+    e0 = narrow_coef(seed, stream, (uint32_t)slot);
+    e1 = narrow_coef(seed, stream, (uint32_t)(slot + ng));
+  } else if constexpr (KIND == CUPOLY_BINARY) {
+    e0 = binary_coef(seed, stream, (uint32_t)slot);
+    e1 = binary_coef(seed, stream, (uint32_t)(slot + ng));
   } else {
     e0 = gaussian_coef(seed, stream, (uint32_t)slot, cdt);
     e1 = gaussian_coef(seed, stream, (uint32_t)(slot + ng), cdt);
@@ -55,6 +74,59 @@ __global__ void __launch_bounds__(256)
     v.re = small_mod(e0, q);
     v.imag = small_mod(e1, q);
     out[(size_t)r * ng + slot] = v;
+  }
+}
+
+// One workgroup: the h coefficients (of n = 2 ng) with the smallest keys
+// become +-1, all others 0; every residue gets the value mod q_r.
+#define FW_THREADS 1024
+__global__ void __launch_bounds__(FW_THREADS)
+    k_sample_fixed_weight(cp_gi *__restrict__ out, int ng, int nres, int roff,
+                          const cp_prime *__restrict__ primes, uint64_t seed, uint64_t stream,
+                          int h) {
+  __shared__ unsigned hist[256];
+  __shared__ uint64_t s_prefix;
+  __shared__ int s_need;
+  const int n = 2 * ng, tid = (int)threadIdx.x;
+  uint64_t prefix = 0;  // top bits of the h-th smallest key, 8 more per pass
+  int need = h;         // its rank among the keys that share those bits
+  if (h > 0 && h < n) {
+    for (int pass = 0; pass < 8; pass++) {
+      const int shift = 56 - 8 * pass;
+      for (int i = tid; i < 256; i += FW_THREADS) hist[i] = 0;
+      __syncthreads();
+      for (int j = tid; j < n; j += FW_THREADS) {
+        bool neg;
+        const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+        if (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
+          atomicAdd(&hist[(key >> shift) & 255], 1u);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int cum = 0, d = 0;
+        for (; d < 255; d++) {
+          if (cum + (int)hist[d] >= need) break;
+          cum += (int)hist[d];
+        }
+        s_need = need - cum;
+        s_prefix = prefix | ((uint64_t)d << shift);
+      }
+      __syncthreads();
+      prefix = s_prefix;
+      need = s_need;
+      __syncthreads();  // hist and the broadcast slots are reused by the next pass
+    }
+  }
+  for (int j = tid; j < n; j += FW_THREADS) {
+    bool neg;
+    const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+    const bool sel = h >= n || (h > 0 && key <= prefix);
+    const int e = sel ? (neg ? -1 : 1) : 0;
+    const int slot = j < ng ? j : j - ng;
+    for (int r = 0; r < nres; r++) {
+      uint64_t *dst = j < ng ? &out[(size_t)r * ng + slot].re : &out[(size_t)r * ng + slot].imag;
+      *dst = small_mod(e, primes[roff + r].p);
+    }
   }
 }
 
@@ -82,6 +154,25 @@ extern "C" int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint
                          dim3(256), 0, c->stream, (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed,
                          stream, cp_make_cdt());
       break;
+    case CUPOLY_NARROW:
+      hipLaunchKernelGGL(k_sample_small<CUPOLY_NARROW>, dim3((ng + 255) / 256), dim3(256), 0,
+                         c->stream, (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream,
+                         cp_make_cdt());
+      break;
+    case CUPOLY_BINARY:
+      hipLaunchKernelGGL(k_sample_small<CUPOLY_BINARY>, dim3((ng + 255) / 256), dim3(256), 0,
+                         c->stream, (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream,
+                         cp_make_cdt());
+      break;
+    case CUPOLY_ZO:
+    case CUPOLY_HAMMINGWEIGHT: {
+      if (2 * ng > 65536) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fixed-weight sampler: n > 65536");
+      const int h = kind == CUPOLY_ZO ? ng : (2 * ng < CUPOLY_HAMMINGWEIGHT_H ? 2 * ng
+                                                                           : CUPOLY_HAMMINGWEIGHT_H);
+      hipLaunchKernelGGL(k_sample_fixed_weight, dim3(1), dim3(FW_THREADS), 0, c->stream,
+                         (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
+      break;
+    }
     default: return cp_fail(CUPOLY_ERR_ARG, "Invalid sampler: " + std::to_string(kind));
   }
   CP_LAUNCH_CHECK();

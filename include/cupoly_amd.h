@@ -43,8 +43,20 @@ enum { CUPOLY_QBASE = 0, CUPOLY_BBASE = 1, CUPOLY_QBBASE = 2, CUPOLY_TBASE = 3 }
 enum { CUPOLY_FORWARD = 0, CUPOLY_INVERSE = 1 };
 /* Scalar ops, settings.h:33 (enum add_mode_t). */
 enum { CUPOLY_ADD = 0, CUPOLY_SUB = 1, CUPOLY_MUL = 2, CUPOLY_MULMUL = 3, CUPOLY_ADDADD = 4 };
-/* Sampler kinds (re-specified on Philox4x32-10; sampler.h:32-41 names). */
-enum { CUPOLY_DISCRETE_GAUSSIAN = 0, CUPOLY_UNIFORM = 5, CUPOLY_TERNARY = 6 };
+/* Sampler kinds: the reference's enum pdistributions (sampler.h:32-41) values,
+ * re-specified on Philox4x32-10, plus TERNARY (uniform {-1, 0, 1}).
+ * HAMMINGWEIGHT draws exactly CUPOLY_HAMMINGWEIGHT_H nonzero +-1 coefficients
+ * (settings.h:43), ZO exactly n/2. */
+enum {
+  CUPOLY_DISCRETE_GAUSSIAN = 0,
+  CUPOLY_HAMMINGWEIGHT = 1,
+  CUPOLY_NARROW = 2,
+  CUPOLY_ZO = 3,
+  CUPOLY_BINARY = 4,
+  CUPOLY_UNIFORM = 5,
+  CUPOLY_TERNARY = 6
+};
+#define CUPOLY_HAMMINGWEIGHT_H 64
 
 const char *cupoly_last_error(void);
 const char *cupoly_version(void);   /* tool/version.h */
@@ -62,7 +74,7 @@ int cupoly_get_n_residues(const cupoly_engine *e, int base);
 int cupoly_engine_info(const cupoly_engine *e, int *n, int *ng, int *k, int *kl, uint64_t *t);
 int cupoly_engine_primes(const cupoly_engine *e, uint64_t *out /* k+kl */);
 /* HPS constants as computed by the engine (CUDAEngine::precompute,
- * cudaengine.cu:1337-1511); for tests.  name: "invQqi" "Qqibi" "Qbi" "invBbi"
+ * cudaengine.cu:1337-1511); for tests.  name: "invQqi" "Qqi" "Qqibi" "Qbi" "invBbi"
  * "Bbiqi" "Bqi" "lambda" "wint" "wfrac" "twint" "twfrac"; returns count. */
 int cupoly_engine_constant(const cupoly_engine *e, const char *name, uint64_t *out, int cap);
 /* The same constants for Q = first k bucket primes, B = next kl, computed on
@@ -136,6 +148,20 @@ int cupoly_sample(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64
 /* Same draws in the coefficient domain (no DGT). */
 int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64_t stream,
                         int base);
+
+/* ---- BFV word decomposition and dot product ------------------------------
+ * poly_xi_bfv -> CUDAEngine::execute_xi_rns (cudaengine.cu:992-1035):
+ * residue i of out[i] = a_i (Q/q_i)^-1 mod q_i for i < k (base Q input).
+ * poly_rho_bfv -> execute_rho_bfv_rns (cudaengine.cu:1037-1062): residue i of
+ * out[i] = a_i (Q/q_i) mod q_i for i < k; a in base Q or QB (B residues are
+ * not used).  Only residue i of each out[i] is written, as in the reference.
+ * out is a host array of k device pointers to polynomials of the input's base. */
+int cupoly_xi_bfv(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *a);
+int cupoly_rho_bfv(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *a, int base);
+/* poly_dot (polynomial.cu:1439-1451): out = sum_{i < count} a[i] * b[i],
+ * slot-wise in the DGT domain of `base`; a, b host arrays of device pointers. */
+int cupoly_dot(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *const *a, const cupoly_gi *const *b,
+               int count, int base);
 
 /* ---- batched products (no reference counterpart: the reference issues one
  * poly_mul per product, 7 launches each, polynomial.cu:892-914) ------------

@@ -38,8 +38,15 @@ enum bases { QBase = CUPOLY_QBASE, BBase = CUPOLY_BBASE, QBBase = CUPOLY_QBBASE,
 enum dgt_direction { FORWARD = CUPOLY_FORWARD, INVERSE = CUPOLY_INVERSE };
 enum add_mode_t { ADD = CUPOLY_ADD, SUB = CUPOLY_SUB, MUL = CUPOLY_MUL, MULMUL = CUPOLY_MULMUL,
                   ADDADD = CUPOLY_ADDADD };
-enum pdistributions { DISCRETE_GAUSSIAN = CUPOLY_DISCRETE_GAUSSIAN, UNIFORM = CUPOLY_UNIFORM,
-                      TERNARY = CUPOLY_TERNARY };
+enum pdistributions {
+  DISCRETE_GAUSSIAN = CUPOLY_DISCRETE_GAUSSIAN,
+  HAMMINGWEIGHT = CUPOLY_HAMMINGWEIGHT,
+  NARROW = CUPOLY_NARROW,
+  ZO = CUPOLY_ZO,
+  BINARY = CUPOLY_BINARY,
+  UNIFORM = CUPOLY_UNIFORM,
+  TERNARY = CUPOLY_TERNARY
+};
 enum poly_states { HOSTSTATE, RNSSTATE, BOTHSTATE };
 
 namespace cupoly_detail {
@@ -245,6 +252,39 @@ inline void poly_dr2(Context *ctx, poly_t *ct21, poly_t *ct22, poly_t *ct23, pol
   cupoly_detail::check(cupoly_dr2_dgt(ctx->ctx, ct21->d_coefs, ct22->d_coefs, ct23->d_coefs,
                                       ct01->d_coefs, ct02->d_coefs, ct11->d_coefs, ct12->d_coefs,
                                       ct01->base));
+}
+// poly_xi_bfv / poly_rho_bfv (polynomial.cu:610-693): c is an array of k
+// polynomials; residue i of c[i] receives the scaled residue i of a.
+inline void poly_xi_bfv(Context *ctx, poly_t *c, poly_t *a) {
+  const int k = CUDAEngine::get_n_residues(QBase);
+  std::vector<cupoly_gi *> out(k);
+  for (int i = 0; i < k; i++) {
+    poly_init(ctx, &c[i], QBase);
+    out[i] = c[i].d_coefs;
+  }
+  cupoly_detail::check(cupoly_xi_bfv(ctx->ctx, out.data(), a->d_coefs));
+  for (int i = 0; i < k; i++) c[i].status = RNSSTATE;
+}
+inline void poly_rho_bfv(Context *ctx, poly_t *c, poly_t *a) {
+  const int k = CUDAEngine::get_n_residues(QBase);
+  std::vector<cupoly_gi *> out(k);
+  for (int i = 0; i < k; i++) {
+    poly_init(ctx, &c[i], a->base);
+    out[i] = c[i].d_coefs;
+  }
+  cupoly_detail::check(cupoly_rho_bfv(ctx->ctx, out.data(), a->d_coefs, a->base));
+  for (int i = 0; i < k; i++) c[i].status = RNSSTATE;
+}
+// poly_dot (polynomial.cu:1439-1451): c = sum_{i < k} a[i] * b[i].
+inline void poly_dot(Context *ctx, poly_t *c, poly_t *a, poly_t *b, const int k) {
+  poly_init(ctx, c, a[0].base);
+  std::vector<const cupoly_gi *> pa(k), pb(k);
+  for (int i = 0; i < k; i++) {
+    pa[i] = a[i].d_coefs;
+    pb[i] = b[i].d_coefs;
+  }
+  cupoly_detail::check(cupoly_dot(ctx->ctx, c->d_coefs, pa.data(), pb.data(), k, a[0].base));
+  c->status = RNSSTATE;
 }
 inline void poly_mul_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
   poly_init(ctx, c, a->base);

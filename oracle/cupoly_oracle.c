@@ -444,6 +444,12 @@ void or_simple_scaling(uint64_t *out, const uint64_t *in, int Ng, int k, uint64_
   }
 }
 
+/* xi / rho of the BFV word decomposition (cudaengine.cu:992-1062,
+ * mulint_dgt): one residue times a constant, both halves of every slot. */
+void or_scale_residue(uint64_t *out, const uint64_t *in, int Ng, uint64_t p, uint64_t c) {
+  for (size_t i = 0; i < (size_t)2 * Ng; i++) out[i] = or_mulmod(in[i], c, p);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as  */
 /* easy as 1, 2, 3", SC'11).                                                */
@@ -517,6 +523,57 @@ int or_sample_gaussian_coef(uint64_t seed, uint64_t stream, uint32_t coef,
   return idx - bound;
 }
 
+/* narrow: uniform {0, 1, 2} (sampler.cu:72-103, BFV mode), tag 4, the
+ * same exact rejection as the ternary draw. */
+int or_sample_narrow_coef(uint64_t seed, uint64_t stream, uint32_t coef) {
+  for (uint32_t att = 0;; att++) {
+    uint32_t w[4];
+    philox_draw(seed, stream, coef, 4, 0, att, w);
+    for (int h = 0; h < 4; h++)
+      if (w[h] < 4294967295u) return (int)(w[h] % 3u);
+  }
+}
+
+/* binary: uniform {0, 1} (sampler.cu:158-182), tag 5. */
+int or_sample_binary_coef(uint64_t seed, uint64_t stream, uint32_t coef) {
+  uint32_t w[4];
+  philox_draw(seed, stream, coef, 5, 0, 0, w);
+  return (int)(w[0] & 1u);
+}
+
+/* Fixed weight h (ZO: h = n/2, sampler.cu:222-278; Hamming weight: h = 64,
+ * sampler.cu:401-446): the nonzero positions are the h coefficients with
+ * the smallest keys, key = 48 random bits (tag 6) above the 16-bit index
+ * (unique), each nonzero +1 or -1 by one more random bit. */
+uint64_t or_fixed_weight_key(uint64_t seed, uint64_t stream, uint32_t coef, int *neg) {
+  uint32_t w[4];
+  philox_draw(seed, stream, coef, 6, 0, 0, w);
+  *neg = (int)(w[2] & 1u);
+  return ((((uint64_t)w[0] << 16) | (w[1] >> 16)) << 16) | (coef & 0xFFFFu);
+}
+
+static int cmp_u64(const void *x, const void *y) {
+  const uint64_t a = *(const uint64_t *)x, b = *(const uint64_t *)y;
+  return a < b ? -1 : a > b;
+}
+
+void or_sample_fixed_weight(int64_t *out, int n, int h, uint64_t seed, uint64_t stream) {
+  uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+  for (int c = 0; c < n; c++) {
+    int neg;
+    keys[c] = or_fixed_weight_key(seed, stream, (uint32_t)c, &neg);
+    out[c] = 0;
+  }
+  qsort(keys, (size_t)n, sizeof(uint64_t), cmp_u64);
+  for (int i = 0; i < h && i < n; i++) {
+    const uint32_t c = (uint32_t)(keys[i] & 0xFFFFu);
+    int neg;
+    (void)or_fixed_weight_key(seed, stream, c, &neg);
+    out[c] = neg ? -1 : 1;
+  }
+  free(keys);
+}
+
 /* Fill flat coefficient arrays for a whole polynomial. */
 void or_sample_uniform_poly(uint64_t *out, int n, int nres, const uint64_t *primes,
                             uint64_t seed, uint64_t stream) {
@@ -530,6 +587,8 @@ void or_sample_small_poly(int64_t *out, int n, int kind, uint64_t seed, uint64_t
                           const uint64_t *cdt, int bound) {
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++)
-    out[c] = kind == 2 ? or_sample_ternary_coef(seed, stream, c)
-                       : or_sample_gaussian_coef(seed, stream, c, cdt, bound);
+    out[c] = kind == 2   ? or_sample_ternary_coef(seed, stream, c)
+             : kind == 4 ? or_sample_narrow_coef(seed, stream, c)
+             : kind == 5 ? or_sample_binary_coef(seed, stream, c)
+                         : or_sample_gaussian_coef(seed, stream, c, cdt, bound);
 }

@@ -61,6 +61,8 @@ def lib():
         L.or_basis_ext_b_to_q.argtypes = [_u64p, _u64p, i32, i32, i32, _u64p, _u64p,
                                           _u64p, _u64p, _u64p, _u64p]
         L.or_simple_scaling.argtypes = [_u64p, _u64p, i32, i32, u64, _u64p, _u64p]
+        L.or_scale_residue.argtypes = [_u64p, _u64p, i32, u64, u64]
+        L.or_sample_fixed_weight.argtypes = [_i64p, i32, i32, u64, u64]
         L.or_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.or_sample_uniform_poly.argtypes = [_u64p, i32, i32, _u64p, u64, u64]
         L.or_sample_small_poly.argtypes = [_i64p, i32, i32, u64, u64, _u64p, i32]
@@ -133,6 +135,7 @@ class Params:
         q, b = self.q, self.b
         # cudaengine.cu:1363-1411
         self.invQqi = [pow((Q // qj) % qj, -1, qj) for qj in q]
+        self.Qqi = [(Q // qj) % qj for qj in q]
         self.Qqibi = [(Q // qj) % bi for bi in b for qj in q]          # [i][j]
         self.Qbi = [Q % bi for bi in b]
         self.invBbi = [pow((B // bi) % bi, -1, bi) for bi in b]
@@ -212,6 +215,36 @@ def polymul_batch(P, a, b, base=0):
     return c
 
 
+def xi_bfv(P, a, outs=None):
+    """poly_xi_bfv restated (cudaengine.cu:992-1035): residue i of output i is
+    a_i (Q/q_i)^-1 mod q_i; other residues keep outs' contents (zeros)."""
+    a = u64(a)
+    outs = [np.zeros_like(a) for _ in range(P.k)] if outs is None else [u64(o).copy() for o in outs]
+    for i in range(P.k):
+        lib().or_scale_residue(_p(outs[i][i]), _p(np.ascontiguousarray(a[i])), P.Ng, P.q[i],
+                               P.invQqi[i])
+    return outs
+
+
+def rho_bfv(P, a, outs=None):
+    """poly_rho_bfv restated (cudaengine.cu:1037-1062): residue i < k of output
+    i is a_i (Q/q_i) mod q_i; a in base Q or QB."""
+    a = u64(a)
+    outs = [np.zeros_like(a) for _ in range(P.k)] if outs is None else [u64(o).copy() for o in outs]
+    for i in range(P.k):
+        lib().or_scale_residue(_p(outs[i][i]), _p(np.ascontiguousarray(a[i])), P.Ng, P.q[i],
+                               P.Qqi[i])
+    return outs
+
+
+def dot(P, a_list, b_list, base=0):
+    """poly_dot (polynomial.cu:1439-1451): poly_mul then poly_mul_add."""
+    acc = pointwise(P, a_list[0], b_list[0], base=base, op=2)
+    for x, y in zip(a_list[1:], b_list[1:]):
+        acc = pointwise(P, x, y, acc, base=base, op=3)
+    return acc
+
+
 def schoolbook(a, b, p):
     a, b = u64(a), u64(b)
     c = np.empty_like(a)
@@ -278,8 +311,16 @@ def sample_uniform(primes, n, seed, stream):
     return out
 
 
+def sample_fixed_weight(n, h, seed, stream):
+    """Exactly min(h, n) coefficients +-1 at the smallest keys.  int64 [n]."""
+    out = np.zeros(n, dtype=np.int64)
+    lib().or_sample_fixed_weight(_p(out, _i64p), n, h, seed, stream)
+    return out
+
+
 def sample_small(n, kind, seed, stream):
-    """kind 2 = ternary, 3 = discrete Gaussian.  Returns int64 [n]."""
+    """kind 2 = ternary, 3 = discrete Gaussian, 4 = narrow {0,1,2},
+    5 = binary {0,1}.  Returns int64 [n]."""
     table, bound, _ = cdt()
     out = np.zeros(n, dtype=np.int64)
     lib().or_sample_small_poly(_p(out, _i64p), n, kind, seed, stream, _p(table), bound)

@@ -56,6 +56,32 @@ int main(int argc, char **argv) {
         return 1;
       }
       Sampler::sample(&ctx, &e, UNIFORM);
+      // word decomposition (test_bfv_mode.cu:492-546): sum_i rho(a)[i] xi(b)[i] = a b,
+      // through poly_dot and through the reference's poly_mul_add loop
+      {
+        const int kk = CUDAEngine::get_n_residues(QBase);
+        poly_t *A = new poly_t[kk];
+        poly_t *B = new poly_t[kk];
+        poly_rho_bfv(&ctx, A, pa);
+        poly_xi_bfv(&ctx, B, pb);
+        poly_t acc, acc2;
+        poly_dot(&ctx, &acc, A, B, kk);
+        poly_init(&ctx, &acc2);
+        for (int i = 0; i < kk; i++) poly_mul_add(&ctx, &acc2, &A[i], &B[i], &acc2);
+        const bool ok = poly_get_residues(&ctx, &acc) == prod && poly_get_residues(&ctx, &acc2) == prod;
+        for (int i = 0; i < kk; i++) {
+          poly_free(&ctx, &A[i]);
+          poly_free(&ctx, &B[i]);
+        }
+        delete[] A;
+        delete[] B;
+        poly_free(&ctx, &acc);
+        poly_free(&ctx, &acc2);
+        if (!ok) {
+          std::fprintf(stderr, "rho/xi/dot mismatch\n");
+          return 1;
+        }
+      }
       poly_free(&ctx, pa);
       poly_free(&ctx, pb);
       poly_free(&ctx, pc);

@@ -65,7 +65,7 @@ def test_null_arguments_are_errors():
 def test_host_constants_match_exact_integers(k, kl, t):
     P = O.Params(k, kl, 2048, t)
     expect = {
-        "invQqi": P.invQqi, "Qqibi": P.Qqibi, "Qbi": P.Qbi, "invBbi": P.invBbi,
+        "invQqi": P.invQqi, "Qqi": P.Qqi, "Qqibi": P.Qqibi, "Qbi": P.Qbi, "invBbi": P.invBbi,
         "Bbiqi": P.Bbiqi, "Bqi": P.Bqi, "lambda": P.lam, "wint": P.wint,
         "twint": P.twint, "twfrac": P.twfrac,
     }

@@ -192,9 +192,15 @@ def test_hps_poly_level_compositions(gpu):
     assert np.array_equal(cp.to_host(dq), aq_dgt)  # inputs preserved
 
 
-@pytest.mark.parametrize("kind", [cp.UNIFORM, cp.TERNARY, cp.DISCRETE_GAUSSIAN])
-def test_sampler_matches_oracle_counters(gpu, kind):
-    n, k = 4096, 3
+SMALL_KIND = {cp.TERNARY: 2, cp.DISCRETE_GAUSSIAN: 3, cp.NARROW: 4, cp.BINARY: 5}
+
+
+@pytest.mark.parametrize("kind,n", [(cp.UNIFORM, 4096), (cp.TERNARY, 4096),
+                                    (cp.DISCRETE_GAUSSIAN, 4096), (cp.NARROW, 4096),
+                                    (cp.BINARY, 4096), (cp.ZO, 4096), (cp.HAMMINGWEIGHT, 4096),
+                                    (cp.ZO, 65536), (cp.HAMMINGWEIGHT, 32), (cp.ZO, 32)])
+def test_sampler_matches_oracle_counters(gpu, kind, n):
+    k = 3 if n <= 16384 else 1
     e, ctx = engine(k, k + 1, n)
     P = O.Params(k, k + 1, n, 65537)
     out = cp.empty((k, P.Ng, 2))
@@ -202,7 +208,12 @@ def test_sampler_matches_oracle_counters(gpu, kind):
     if kind == cp.UNIFORM:
         want = O.sample_uniform(P.q, n, 777, 5)
     else:
-        vals = O.sample_small(n, 2 if kind == cp.TERNARY else 3, 777, 5)
+        if kind in (cp.ZO, cp.HAMMINGWEIGHT):
+            h = n // 2 if kind == cp.ZO else min(cp.HAMMINGWEIGHT_H, n)
+            vals = O.sample_fixed_weight(n, h, 777, 5)
+            assert int(np.count_nonzero(vals)) == h
+        else:
+            vals = O.sample_small(n, SMALL_KIND[kind], 777, 5)
         want = O.small_to_residues(vals, P.q)
     assert np.array_equal(O.unfold(cp.to_host(out)), want)
     # DGT-domain variant = forward DGT of the same draws

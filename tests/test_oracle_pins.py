@@ -268,6 +268,29 @@ def test_pointwise_matches_gaussian_integer_arithmetic():
                 assert (int(got[op][r, i, 0]), int(got[op][r, i, 1])) == want[op], (op, r, i)
 
 
+def test_xi_rho_dot_word_decomposition():
+    """xi / rho against Python-integer products by (Q/q_i)^-1 and Q/q_i, only
+    residue i of output i touched, and the reference's identity
+    sum_i rho(a)[i] * xi(b)[i] == a * b (test_bfv_mode.cu:492-546) through dot."""
+    P = O.Params(3, 4, 256, 65537)
+    rng = np.random.default_rng(3)
+    a = np.stack([rng.integers(0, q, size=(P.Ng, 2), dtype=np.uint64) for q in P.q])
+    b = np.stack([rng.integers(0, q, size=(P.Ng, 2), dtype=np.uint64) for q in P.q])
+    X = O.xi_bfv(P, b)
+    R = O.rho_bfv(P, a)
+    for i, q in enumerate(P.q):
+        Qi = P.Q // q
+        assert P.Qqi[i] == Qi % q and P.invQqi[i] * Qi % q == 1
+        for j in range(P.k):
+            if j != i:
+                assert not X[i][j].any() and not R[i][j].any()
+        for s_ in range(0, P.Ng, 17):
+            for h in range(2):
+                assert int(X[i][i, s_, h]) == int(b[i, s_, h]) * pow(Qi, -1, q) % q
+                assert int(R[i][i, s_, h]) == int(a[i, s_, h]) * Qi % q
+    assert np.array_equal(O.dot(P, R, X), O.pointwise(P, a, b, op=2))
+
+
 def test_dgt_roundtrip_and_constant():
     P = O.Params(3, 4, 4096, 65537)
     rng = np.random.default_rng(3)
@@ -486,6 +509,46 @@ def test_ternary_chi_square():
     counts = [int(np.sum(vals == v)) for v in (-1, 0, 1)]
     # 2 degrees of freedom: P(chi2 > 13.8) = 0.001
     assert chi2_stat(counts, [1 / 3] * 3) < 13.8
+
+
+def test_narrow_and_binary_chi_square():
+    vals = O.sample_small(1 << 16, 4, 98, 3)
+    assert vals.min() == 0 and vals.max() == 2
+    assert chi2_stat([int(np.sum(vals == v)) for v in (0, 1, 2)], [1 / 3] * 3) < 13.8
+    vals = O.sample_small(1 << 16, 5, 97, 3)
+    assert set(np.unique(vals)) == {0, 1}
+    assert chi2_stat([int(np.sum(vals == v)) for v in (0, 1)], [0.5, 0.5]) < 10.8  # df 1
+
+
+def test_fixed_weight_exact_count_uniform_positions_and_signs():
+    """ZO (h = n/2) and Hamming weight (h = 64): exactly h nonzeros of +-1;
+    over many streams each position is chosen with probability h/n and the
+    signs are fair (chi-square, p = 0.001); the chosen set equals a brute-force
+    selection of the h smallest keys."""
+    import ctypes
+    L = O.lib()
+    L.or_fixed_weight_key.restype = ctypes.c_uint64
+    L.or_fixed_weight_key.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_int)]
+    n = 256
+    for h in (64, 128):
+        hits = np.zeros(n, dtype=np.int64)
+        pos = neg = 0
+        for stream in range(800):
+            v = O.sample_fixed_weight(n, h, 4242, stream)
+            assert int(np.count_nonzero(v)) == h and set(np.unique(v)) <= {-1, 0, 1}
+            hits += v != 0
+            pos += int(np.sum(v == 1))
+            neg += int(np.sum(v == -1))
+        # df 255: 0.999 quantile ~ 330; positions are not independent across
+        # coefficients of one draw, but the sum over draws is multinomial-like
+        assert chi2_stat(list(hits), [1 / n] * n) < 330
+        assert chi2_stat([pos, neg], [0.5, 0.5]) < 10.8
+    neg_ = ctypes.c_int()
+    keys = [(L.or_fixed_weight_key(1, 2, j, ctypes.byref(neg_)), j) for j in range(512)]
+    assert len({k for k, _ in keys}) == 512  # distinct
+    chosen = {j for _, j in sorted(keys)[:64]}
+    assert chosen == set(np.nonzero(O.sample_fixed_weight(512, 64, 1, 2))[0].tolist())
 
 
 def test_gaussian_chi_square_and_bound():
