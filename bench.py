@@ -26,6 +26,71 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 
 
+def bench_edges(n=16384, k=4, reps=100):
+    """Per-call device time of the coefficient-edge and key-switching ops at
+    (n, k): poly_crt / poly_icrt, Q->t scaling (t = 65537), xi / rho / dot
+    (k pairs), and the fixed-weight / narrow / binary samplers; with the HBM
+    bytes each moves."""
+    import numpy as np
+    import torch
+    import cupoly_amd as cp
+
+    eng = cp.CUDAEngine(k, k + 1, n, 65537)
+    stream = torch.cuda.Stream()
+    ctx = cp.Context(eng, stream=stream)
+    ng = n // 2
+    res = cp.empty((k, ng, 2))
+    with torch.cuda.stream(stream):
+        cp.Sampler.sample(ctx, res, cp.UNIFORM, 5, 1, coeff_domain=True)
+    ints = cp.empty((n, k))
+    gi = cp.empty((k, ng, 2))
+    outs = [cp.empty((k, ng, 2)) for _ in range(k)]
+    A = [cp.empty((k, ng, 2)) for _ in range(k)]
+    B = [cp.empty((k, ng, 2)) for _ in range(k)]
+    for i in range(k):
+        with torch.cuda.stream(stream):
+            cp.Sampler.sample(ctx, A[i], cp.UNIFORM, 6, i)
+            cp.Sampler.sample(ctx, B[i], cp.UNIFORM, 7, i)
+    sc = cp.empty((k, ng, 2))
+    stream.synchronize()
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / reps  # us
+
+    P8 = 8 * k * n  # bytes of one polynomial's residues
+    ops = [
+        ("icrt", lambda: cp.icrt(ctx, ints, res), P8 + 8 * n * k),
+        ("crt", lambda: cp.crt(ctx, gi, ints), 8 * n * k + P8),
+        ("simple_scaling_t65537", lambda: (sc.copy_(res), cp.execute_polynomial_simple_scaling(ctx, sc)), P8 + 8 * n),
+        ("xi", lambda: cp.execute_xi_bfv(ctx, outs, res), 2 * P8),
+        ("rho", lambda: cp.execute_rho_bfv(ctx, outs, res), 2 * P8),
+        ("dot_k_pairs", lambda: cp.dot(ctx, gi, A, B), (2 * k + 1) * P8),
+        ("sample_zo", lambda: cp.Sampler.sample(ctx, gi, cp.ZO, 9, 1, coeff_domain=True), P8),
+        ("sample_hamming_weight", lambda: cp.Sampler.sample(ctx, gi, cp.HAMMINGWEIGHT, 9, 2, coeff_domain=True), P8),
+        ("sample_narrow", lambda: cp.Sampler.sample(ctx, gi, cp.NARROW, 9, 3, coeff_domain=True), P8),
+        ("sample_binary", lambda: cp.Sampler.sample(ctx, gi, cp.BINARY, 9, 4, coeff_domain=True), P8),
+    ]
+    out = {"n": n, "k": k, "unit": "us per call (one polynomial)"}
+    for name, fn, nbytes in ops:
+        with torch.cuda.stream(stream):
+            us = timed(fn)
+        out[name] = {"us": us, "bytes": nbytes, "GBs": nbytes / (us * 1e-6) / 1e9}
+    # icrt/crt round trip equals the input residues
+    with torch.cuda.stream(stream):
+        cp.icrt(ctx, ints, res)
+        cp.crt(ctx, gi, ints)
+    stream.synchronize()
+    assert torch.equal(gi, res), "crt(icrt(x)) != x"
+    return out
+
+
 def bench_uneven(count, steps, seed=4096):
     """`count` products, n uniform in {2^11..2^15}, k uniform in 1..8, uniform
     residues from the Philox sampler; persistent one-launch form (graph replay)
@@ -122,6 +187,8 @@ def main():
                     help="offered load of the serving run, requests/s (Poisson)")
     ap.add_argument("--uneven-count", type=int, default=4096,
                     help="products in the uneven batch (n in 2^11..2^15, k in 1..8); 0 skips")
+    ap.add_argument("--edges", type=int, default=1,
+                    help="1: also time the coefficient-edge / key-switching / sampler ops")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -258,6 +325,11 @@ def main():
         print("bench: uneven done %.0f products/s (%.1fx plain)"
               % (uneven["value"], uneven["speedup_vs_plain"]), file=sys.stderr, flush=True)
 
+    edges = bench_edges(n, k) if args.edges else None
+    if edges:
+        print("bench: edges done (icrt %.1f us, crt %.1f us)" % (edges["icrt"]["us"], edges["crt"]["us"]),
+              file=sys.stderr, flush=True)
+
     # serving: Poisson arrivals, 70% (n=2^13, k=3) / 30% (n=2^15, k=8) BFV
     # multiplications, continuous batching, HBM-footprint admission
     serve = None
@@ -330,6 +402,7 @@ def main():
         "sample_muladd": enc,
         "serving": serve,
         "uneven": uneven,
+        "edges": edges,
     }
     print(json.dumps(line), flush=True)
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
+    "cupoly_crt": ([_vp, _vp, _vp, _int, _int, _int], _int),
+    "cupoly_icrt": ([_vp, _vp, _vp], _int),
     "cupoly_xi_bfv": ([_vp, _vp, _vp], _int),
     "cupoly_rho_bfv": ([_vp, _vp, _vp, _int], _int),
     "cupoly_dot": ([_vp, _vp, _vp, _vp, _int, _int], _int),
@@ -374,6 +376,45 @@ def sample_muladd_batch(ctx, out, a, seed, stream0):
     _need(a, e.k, e.N, batch)
     _need(out, e.k, e.N, batch)
     _check(lib().cupoly_sample_muladd_batch(ctx.handle, _ptr(out), _ptr(a), batch, seed, stream0))
+
+
+def crt(ctx, out, coefs, base=QBase):
+    """poly_crt on the device: coefs int64 tensor [ncoefs, limbs] holding the
+    little-endian 64-bit limbs of unsigned coefficients -> out [nres, n/2, 2]
+    residues (coefficient domain)."""
+    e = ctx.engine
+    _need(out, e.get_n_residues(base), e.N)
+    if coefs.dim() != 2 or coefs.shape[0] > 2 * e.N:
+        raise ValueError("cupoly_amd: crt coefficients must be [ncoefs <= n, limbs]")
+    _check(lib().cupoly_crt(ctx.handle, _ptr(out), _ptr(coefs), coefs.shape[0], coefs.shape[1],
+                            base))
+
+
+def icrt(ctx, out, residues):
+    """poly_icrt on the device: residues [k, n/2, 2] (base Q, coefficient
+    domain) -> out int64 tensor [n, k] of little-endian limbs of x mod Q."""
+    e = ctx.engine
+    _need(residues, e.k, e.N)
+    if tuple(out.shape) != (2 * e.N, e.k):
+        raise ValueError("cupoly_amd: icrt output must be [n, k]")
+    _check(lib().cupoly_icrt(ctx.handle, _ptr(out), _ptr(residues)))
+
+
+def limbs_to_ints(arr):
+    """numpy uint64 [m, L] -> list of Python ints."""
+    import numpy as np
+    a = np.asarray(arr, dtype=np.uint64)
+    return [sum(int(a[i, l]) << (64 * l) for l in range(a.shape[1])) for i in range(a.shape[0])]
+
+
+def ints_to_limbs(vals, limbs):
+    """list of non-negative Python ints -> numpy uint64 [m, limbs]."""
+    import numpy as np
+    out = np.zeros((len(vals), limbs), dtype=np.uint64)
+    for i, v in enumerate(vals):
+        for l in range(limbs):
+            out[i, l] = (v >> (64 * l)) & ((1 << 64) - 1)
+    return out
 
 
 def _ptr_array(ts):

@@ -77,6 +77,34 @@ static int size_index(int ng) {
   return -1;
 }
 
+// Multi-word (little-endian 64-bit limbs) constants of the CRT of base Q.
+static void mp_mul_u64(uint64_t *x, int len, uint64_t m) {
+  cp_u128 carry = 0;
+  for (int l = 0; l < len; l++) {
+    const cp_u128 v = (cp_u128)x[l] * m + carry;
+    x[l] = (uint64_t)v;
+    carry = v >> 64;
+  }
+}
+
+static void compute_crt(cp_crt &c, const uint64_t *primes, int k, int nres) {
+  memset(&c, 0, sizeof(c));
+  c.k = k;
+  c.nres = nres;
+  c.Q[0] = 1;
+  for (int j = 0; j < k; j++) mp_mul_u64(c.Q, CP_MAX_BASE + 1, primes[j]);
+  for (int i = 0; i < k; i++) {
+    c.M[i][0] = 1;
+    for (int j = 0; j < k; j++)
+      if (j != i) mp_mul_u64(c.M[i], CP_MAX_BASE, primes[j]);
+    c.rcp_q[i] = 1.0 / (double)primes[i];
+  }
+  for (int r = 0; r < nres; r++) {
+    const uint64_t p = primes[r];
+    shoup_pair((uint64_t)(((cp_u128)1 << 64) % p), p, c.two64[r]);
+  }
+}
+
 static void compute_hps(cp_hps &h, const uint64_t *primes, int k, int kl, uint64_t t) {
   memset(&h, 0, sizeof(h));
   h.k = k;
@@ -176,6 +204,7 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
     e->primes.push_back(p);
   }
   compute_hps(e->h_hps, e->primes.data(), k, kl, t);
+  compute_crt(e->h_crt, e->primes.data(), k, e->nres);
   hipError_t err = hipGetDevice(&e->device);
   if (err != hipSuccess) {
     delete e;
@@ -232,7 +261,8 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
       (rc = up((void **)&e->d_tw_inv, twi.data(), twi.size() * 8)) ||
       (rc = up((void **)&e->d_twist_fwd, tsf.data(), tsf.size() * 8)) ||
       (rc = up((void **)&e->d_twist_inv, tsi.data(), tsi.size() * 8)) ||
-      (rc = up((void **)&e->d_hps, &e->h_hps, sizeof(cp_hps))) || (rc = cp_dgt_setup(e))) {
+      (rc = up((void **)&e->d_hps, &e->h_hps, sizeof(cp_hps))) ||
+      (rc = up((void **)&e->d_crt, &e->h_crt, sizeof(cp_crt))) || (rc = cp_dgt_setup(e))) {
     std::string msg = g_last_error;
     cupoly_engine_destroy(e);
     return cp_fail(rc, msg);
@@ -245,7 +275,7 @@ extern "C" int cupoly_engine_destroy(cupoly_engine *e) {
   if (!e) return cp_fail(CUPOLY_ERR_ARG, "engine_destroy: null");
   (void)hipDeviceSynchronize();
   void *ptrs[] = {e->d_primes, e->d_tw_fwd, e->d_tw_inv, e->d_twist_fwd, e->d_twist_inv,
-                  e->d_hps};
+                  e->d_hps, e->d_crt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete e;

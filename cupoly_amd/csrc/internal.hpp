@@ -32,6 +32,17 @@ struct cp_hps {
   uint64_t twfrac[CP_MAX_BASE][2];               // simple scaling, fraction * 2^128
 };
 
+// CRT constants of base Q for reconstruction, and 2^64 mod p for every
+// residue (multi-word reduction).  Multi-word values are little-endian
+// 64-bit limbs.
+struct cp_crt {
+  int k, nres;
+  uint64_t Q[CP_MAX_BASE + 1];                // Q (k limbs used, the rest 0)
+  uint64_t M[CP_MAX_BASE][CP_MAX_BASE];       // [i] Q / q_i
+  double rcp_q[CP_MAX_BASE];                  // RN(1 / q_i)
+  uint64_t two64[2 * CP_MAX_BASE][2];         // [r] 2^64 mod p_r, Shoup pair
+};
+
 struct cupoly_engine {
   int k = 0, kl = 0, n = 0, ng = 0, logng = 0, nres = 0;
   uint64_t t = 0;
@@ -45,6 +56,8 @@ struct cupoly_engine {
   uint64_t *d_twist_inv = nullptr;   // [nres][ng][6]     xi^-n / ng
   cp_hps h_hps;                      // host copy
   cp_hps *d_hps = nullptr;
+  cp_crt h_crt;                      // host copy
+  cp_crt *d_crt = nullptr;
 };
 
 struct cupoly_ctx {

@@ -149,6 +149,17 @@ int cupoly_sample(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64
 int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint64_t seed, uint64_t stream,
                         int base);
 
+/* ---- coefficient edges: poly_crt / poly_icrt on the device ------------------
+ * poly_crt (polynomial.cu:190-219): unsigned multi-word coefficients,
+ * little-endian 64-bit limbs, coefs[ncoefs][limbs] (ncoefs <= n; missing high
+ * coefficients are 0) -> residues of `base` (Q, B or QB) in the coefficient-
+ * domain layout [nres][n/2] GIs (coefficient j in re of slot j, j + n/2 in
+ * imag).  poly_icrt (polynomial.cu:221-291) for base Q: residues [k][n/2] GIs
+ * -> x mod Q in [0, Q) as out[n][k] limbs. */
+int cupoly_crt(cupoly_ctx *c, cupoly_gi *out, const uint64_t *coefs, int ncoefs, int limbs,
+               int base);
+int cupoly_icrt(cupoly_ctx *c, uint64_t *out, const cupoly_gi *in);
+
 /* ---- BFV word decomposition and dot product ------------------------------
  * poly_xi_bfv -> CUDAEngine::execute_xi_rns (cudaengine.cu:992-1035):
  * residue i of out[i] = a_i (Q/q_i)^-1 mod q_i for i < k (base Q input).

@@ -390,6 +390,50 @@ inline poly_t *poly_import_residues(Context *ctx, const std::string &s, int base
   return poly_import_residues(ctx, v, base);
 }
 
+// Host coefficient integers in and out (the reference keeps NTL ZZ
+// coefficients in poly_t::coefs and converts with poly_crt / poly_icrt,
+// polynomial.cu:190-291): little-endian 64-bit limbs, `limbs` per
+// coefficient in, k per coefficient out (values in [0, Q)); the conversion
+// runs on the device.
+inline poly_t *poly_import_integers(Context *ctx, const std::vector<uint64_t> &coefs, int limbs,
+                                    int base = QBase) {
+  poly_t *p = new poly_t;
+  poly_init(ctx, p, base);
+  const int ncoefs = (int)(coefs.size() / limbs);
+  uint64_t *d = nullptr;
+  cupoly_detail::hip_check(hipMalloc((void **)&d, coefs.size() * 8 + 8), "poly_import_integers");
+  cupoly_detail::hip_check(hipMemcpyAsync(d, coefs.data(), coefs.size() * 8,
+                                          hipMemcpyHostToDevice, ctx->get_stream()),
+                           "poly_import_integers");
+  cupoly_detail::check(cupoly_crt(ctx->ctx, p->d_coefs, d, ncoefs, limbs, base));
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, base, FORWARD));
+  ctx->sync();
+  (void)hipFree(d);
+  p->status = RNSSTATE;
+  return p;
+}
+inline std::vector<uint64_t> poly_get_integers(Context *ctx, poly_t *a) {
+  const int k = CUDAEngine::get_n_residues(QBase), N = CUDAEngine::N;
+  GaussianInteger *tmp = nullptr;
+  uint64_t *d = nullptr;
+  const size_t bytes = poly_get_residues_size(QBase), obytes = (size_t)2 * N * k * 8;
+  cupoly_detail::hip_check(hipMalloc((void **)&tmp, bytes), "poly_get_integers");
+  cupoly_detail::hip_check(hipMalloc((void **)&d, obytes), "poly_get_integers");
+  cupoly_detail::hip_check(
+      hipMemcpyAsync(tmp, a->d_coefs, bytes, hipMemcpyDeviceToDevice, ctx->get_stream()),
+      "poly_get_integers");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, tmp, QBase, INVERSE));
+  cupoly_detail::check(cupoly_icrt(ctx->ctx, d, tmp));
+  std::vector<uint64_t> out((size_t)2 * N * k);
+  cupoly_detail::hip_check(
+      hipMemcpyAsync(out.data(), d, obytes, hipMemcpyDeviceToHost, ctx->get_stream()),
+      "poly_get_integers");
+  ctx->sync();
+  (void)hipFree(tmp);
+  (void)hipFree(d);
+  return out;
+}
+
 inline bool poly_are_equal(Context *ctx, poly_t *a, poly_t *b) {
   if (a->base != b->base) return false;
   return poly_get_residues(ctx, a) == poly_get_residues(ctx, b);

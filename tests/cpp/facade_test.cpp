@@ -56,6 +56,18 @@ int main(int argc, char **argv) {
         return 1;
       }
       Sampler::sample(&ctx, &e, UNIFORM);
+      // integer coefficients through the device CRT: x mod Q round-trips
+      {
+        const std::vector<uint64_t> xs = poly_get_integers(&ctx, &c);
+        poly_t *pi = poly_import_integers(&ctx, xs, k);
+        const bool ok = poly_are_equal(&ctx, pi, &c);
+        poly_free(&ctx, pi);
+        delete pi;
+        if (!ok) {
+          std::fprintf(stderr, "integer round trip mismatch\n");
+          return 1;
+        }
+      }
       // word decomposition (test_bfv_mode.cu:492-546): sum_i rho(a)[i] xi(b)[i] = a b,
       // through poly_dot and through the reference's poly_mul_add loop
       {

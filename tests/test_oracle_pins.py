@@ -291,6 +291,16 @@ def test_xi_rho_dot_word_decomposition():
     assert np.array_equal(O.dot(P, R, X), O.pointwise(P, a, b, op=2))
 
 
+def test_crt_reconstruction_identity():
+    """oracle CRT (poly_icrt restated) returns x for the residues of any
+    x in [0, Q), including 0 and Q - 1 (Chinese remainder theorem)."""
+    for k in (1, 3, 8):
+        P = O.Params(k, 1, 64, 65537)
+        rng = random.Random(k)
+        xs = [0, P.Q - 1] + [rng.randrange(P.Q) for _ in range(P.n - 2)]
+        assert O.crt(P, residues_of(xs, P.q, P.Ng)) == xs
+
+
 def test_dgt_roundtrip_and_constant():
     P = O.Params(3, 4, 4096, 65537)
     rng = np.random.default_rng(3)
