@@ -18,6 +18,7 @@
 //   5. k_scale_to_q:    complex scaling t/Q (QB -> B) and B -> Q fused in
 //                       registers, one thread per slot, writing base Q.
 // Scratch lives in a grow-only workspace owned by the context.
+#include "dgt_dev.hpp"
 #include "hps_dev.hpp"
 
 // Where ciphertext i lives: base + i * stride, or table[i] (gathered
@@ -67,17 +68,186 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// blockIdx.y = output polynomial index (pair * 3 + m)
+// Complex scaling t/Q (QB -> B) and B -> Q in registers, K = k known at
+// compile time.  blockIdx.y = output polynomial index (pair * 3 + m).  The
+// fraction products x_j omega_frac_j and their FP64 sum do not depend on the
+// B residue i and are formed once; the B residues are produced one at a time
+// and folded straight into the B -> Q sums (ascending i, as the reference's
+// loops), so only K-sized arrays stay live.  Same operations, same order and
+// therefore the same values as hps_scale_slot followed by hps_b_to_q_slot.
+template <int K>
 __global__ void __launch_bounds__(256)
     k_scale_to_q(cp_cts out, const cp_gi *__restrict__ Y, int ng, const cp_hps *__restrict__ H) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
-  const int k = H->k, kl = H->kl;
+  const int kl = H->kl;
   const int pair = blockIdx.y / 3, m = blockIdx.y % 3;
-  const cp_gi *y = Y + (size_t)blockIdx.y * (k + kl) * ng;
-  uint64_t yr[CP_MAX_BASE], yi[CP_MAX_BASE];
-  hps_scale_slot(y, ng, y + (size_t)k * ng, ng, c, H, yr, yi);
-  hps_b_to_q_slot(yr, yi, out.at(pair) + (size_t)m * k * ng, ng, c, H);
+  const cp_gi *y = Y + (size_t)blockIdx.y * (K + kl) * ng;
+  const cp_gi *yb = y + (size_t)K * ng;
+  cp_gi x[K];
+  uint64_t fr[K], fi[K];
+  const double two_m64 = 0x1p-64;
+  double acc_r = 0.0, acc_i = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    x[j] = y[(size_t)j * ng + c];
+    uint64_t lr, li;
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, fr[j]);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, fi[j]);
+    acc_r = fma((double)lr, two_m64, acc_r);
+    acc_i = fma((double)li, two_m64, acc_i);
+  }
+  const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
+  uint64_t sr[K], si[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) sr[j] = si[j] = 0;
+  double vr = 0.0, vi = 0.0;
+  for (int i = 0; i < kl; i++) {
+    const uint64_t b = H->b[i].p;
+    uint64_t rr = 0, ri = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      rr = addmod_ref(rr, addmod_ref(shoup2(x[j].re, H->wint[i][j], b), fr[j], b), b);
+      ri = addmod_ref(ri, addmod_ref(shoup2(x[j].imag, H->wint[i][j], b), fi[j], b), b);
+    }
+    rr = addmod_ref(rr, round_r, b);
+    ri = addmod_ref(ri, round_i, b);
+    const cp_gi yv = yb[(size_t)i * ng + c];
+    rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
+    ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+    // B -> Q, term i (cudaengine.cu:688-703)
+    const uint64_t ar = shoup2(rr, H->invBbi[i], b), ai = shoup2(ri, H->invBbi[i], b);
+    const double bd = (double)b;
+    vr = vr + (double)ar / bd;
+    vi = vi + (double)ai / bd;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint64_t q = H->q[j].p;
+      sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
+      si[j] = addmod_ref(si[j], shoup2(ai, H->Bbiqi[j][i], q), q);
+    }
+  }
+  const uint64_t v_re = snap(vr), v_im = snap(vi);
+  cp_gi *o = out.at(pair) + (size_t)m * K * ng;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const uint64_t q = H->q[j].p;
+    cp_gi r;
+    r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
+    r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
+    o[(size_t)j * ng + c] = r;
+  }
+}
+
+// Tensor product fused into the inverse transform (n <= 2^14): one
+// workgroup row per output residue polynomial (pair, m, r) reads the DGT-
+// domain operands X[pair][0..3][r] in natural order, forms
+// m = 0: c0 d0,  m = 1: c0 d1 + c1 d0,  m = 2: c1 d1
+// slot-wise on its way into LDS, then runs the inverse DGT and untwist.
+template <int LOGN, int ROWS>
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+    k_tensor_idgt(cp_gi *__restrict__ Y, const cp_gi *__restrict__ X, int nunits, int nres,
+                  const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twi_all,
+                  const uint64_t *__restrict__ tsi_all) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  using LP = LastPass<LOGN, 0, LOGN>;
+  extern __shared__ cp_gi lds_all[];
+  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
+  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
+  cp_gi *lds = lds_all + row * lds_row<LOGN>();
+  int unit = blockIdx.x * ROWS + row;  // (pair * 3 + m) * nres + r
+  const bool active = unit < nunits;
+  if (!active) unit = nunits - 1;
+  int rid = unit % nres;
+  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
+  const int pm = unit / nres, pair = pm / 3, m = pm % 3;
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const size_t per_pair = (size_t)nres * N;
+  const cp_gi *x = X + (size_t)pair * 4 * per_pair + (size_t)rid * N;
+  const cp_gi *xa = x + (m == 2 ? per_pair : 0), *xb = x + (m == 0 ? 2 : 3) * per_pair;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = nat_index<LOGN>(t, i);
+    cp_gi v = cp_gimul(xa[g], xb[g], pr);
+    if (m == 1) {  // + c1 d0
+      const cp_gi w = cp_gimul(x[per_pair + g], x[2 * per_pair + g], pr);
+      v.re = cp_add(v.re, w.re, p);
+      v.imag = cp_add(v.imag, w.imag, p);
+    }
+    lds[lds_phys(g, LP::MS)] = v;
+  }
+  __syncthreads();
+  cp_gi v[EPT];
+  lds_load<LOGN, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, v, t);
+  const uint64_t *twi = twi_all + (size_t)rid * N, *tsi = tsi_all + (size_t)rid * N * 6;
+  MapIdentity id;
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(v, lds, t, twi, p, id);
+  if (active) {
+    cp_gi *y = Y + (size_t)unit * N;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = first_index<LOGN, 0, LOGN>(t, i);
+      twist(v[i], tsi, g, p);
+      y[g] = v[i];
+    }
+  }
+}
+
+template <int LOGN>
+static int launch_tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
+  constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
+  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
+  static bool attr = false;
+  if (!attr) {
+    CP_HIP(hipFuncSetAttribute((const void *)k_tensor_idgt<LOGN, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  cupoly_engine *e = c->e;
+  hipLaunchKernelGGL((k_tensor_idgt<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
+                     c->stream, Y, X, nunits, e->nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
+static int tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
+  switch (c->e->logng) {
+    case 10: return launch_tensor_idgt<10>(c, Y, X, nunits);
+    case 11: return launch_tensor_idgt<11>(c, Y, X, nunits);
+    case 12: return launch_tensor_idgt<12>(c, Y, X, nunits);
+    case 13: return launch_tensor_idgt<13>(c, Y, X, nunits);
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "tensor_idgt: ring degree");
+  }
+}
+
+template <int K>
+static void launch_scale(cupoly_ctx *c, cp_cts out, const cp_gi *Y, int batch) {
+  const int ng = c->e->ng;
+  hipLaunchKernelGGL(k_scale_to_q<K>, dim3((ng + 255) / 256, batch * 3), dim3(256), 0, c->stream,
+                     out, Y, ng, c->e->d_hps);
+}
+
+static int scale_to_q(cupoly_ctx *c, cp_cts out, const cp_gi *Y, int batch) {
+  switch (c->e->k) {
+    case 1: launch_scale<1>(c, out, Y, batch); break;
+    case 2: launch_scale<2>(c, out, Y, batch); break;
+    case 3: launch_scale<3>(c, out, Y, batch); break;
+    case 4: launch_scale<4>(c, out, Y, batch); break;
+    case 5: launch_scale<5>(c, out, Y, batch); break;
+    case 6: launch_scale<6>(c, out, Y, batch); break;
+    case 7: launch_scale<7>(c, out, Y, batch); break;
+    case 8: launch_scale<8>(c, out, Y, batch); break;
+    case 9: launch_scale<9>(c, out, Y, batch); break;
+    case 10: launch_scale<10>(c, out, Y, batch); break;
+    case 11: launch_scale<11>(c, out, Y, batch); break;
+    case 12: launch_scale<12>(c, out, Y, batch); break;
+    case 13: launch_scale<13>(c, out, Y, batch); break;
+    case 14: launch_scale<14>(c, out, Y, batch); break;
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "hom_mul: k outside 1..14");
+  }
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
 }
 
 int cp_workspace(cupoly_ctx *c, size_t bytes, void **out) {
@@ -109,17 +279,18 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
                      ng, e->d_hps);
   CP_LAUNCH_CHECK();
   if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
-  const long total = (long)batch * nres * ng;
-  long g = (total + 255) / 256;
-  if (g > 4096) g = 4096;
-  hipLaunchKernelGGL(k_tensor, dim3((unsigned)g), dim3(256), 0, c->stream, Y, X, ng, nres, total,
-                     e->d_primes);
-  CP_LAUNCH_CHECK();
-  if ((rc = cp_dgt_launch(c, Y, 0, nres, batch * 3, CUPOLY_INVERSE))) return rc;
-  hipLaunchKernelGGL(k_scale_to_q, dim3(sg.x, batch * 3), dim3(256), 0, c->stream, out, Y, ng,
-                     e->d_hps);
-  CP_LAUNCH_CHECK();
-  return CUPOLY_OK;
+  if (e->logng >= 10 && e->logng <= 13) {
+    if ((rc = tensor_idgt(c, Y, X, batch * 3 * nres))) return rc;
+  } else {
+    const long total = (long)batch * nres * ng;
+    long g = (total + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_tensor, dim3((unsigned)g), dim3(256), 0, c->stream, Y, X, ng, nres,
+                       total, e->d_primes);
+    CP_LAUNCH_CHECK();
+    if ((rc = cp_dgt_launch(c, Y, 0, nres, batch * 3, CUPOLY_INVERSE))) return rc;
+  }
+  return scale_to_q(c, out, Y, batch);
 }
 
 extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,

@@ -90,6 +90,11 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
 
 // Complex scaling t/Q for slot c: aq[j*qs + c] (j < k), ab[i*bs + c] (i < kl)
 // -> yr[i], yi[i] (residues mod b_i of the re / imag coefficient).
+// omega_frac_j does not depend on the output residue i (it is t (QB/q_j)^-1
+// mod q_j B / q_j, reduced to its fraction), so the 192-bit products x_j
+// omega_frac_j and the FP64 sum of their low words are formed once per slot
+// and reused for every i; the sum runs over j in the same order as the
+// reference's per-(i, slot) thread, so it is the same double.
 __device__ __forceinline__ void hps_scale_slot(const cp_gi *__restrict__ aq, size_t qs,
                                                const cp_gi *__restrict__ ab, size_t bs, int c,
                                                const cp_hps *__restrict__ H,
@@ -97,32 +102,37 @@ __device__ __forceinline__ void hps_scale_slot(const cp_gi *__restrict__ aq, siz
                                                uint64_t (&yi)[CP_MAX_BASE]) {
   const int k = H->k, kl = H->kl;
   cp_gi x[CP_MAX_BASE];
-#pragma unroll
-  for (int j = 0; j < CP_MAX_BASE; j++)
-    if (j < k) x[j] = aq[(size_t)j * qs + c];
+  uint64_t hr[CP_MAX_BASE], hi[CP_MAX_BASE];
   const double two_m64 = 0x1p-64;  // __drcp_rn(2^64)
+  double acc_r = 0.0, acc_i = 0.0;
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      x[j] = aq[(size_t)j * qs + c];
+      uint64_t lr, li;
+      umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, hr[j]);
+      umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, hi[j]);
+      acc_r = fma((double)lr, two_m64, acc_r);  // __fma_rn (cudaengine.cu:413-414)
+      acc_i = fma((double)li, two_m64, acc_i);
+    }
+  }
+  const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
 #pragma unroll
   for (int i = 0; i < CP_MAX_BASE; i++) {
     if (i < kl) {
       const uint64_t b = H->b[i].p;
-      double acc_r = 0.0, acc_i = 0.0;
       uint64_t rr = 0, ri = 0;
 #pragma unroll
       for (int j = 0; j < CP_MAX_BASE; j++) {
         if (j < k) {
           const uint64_t int_r = shoup2(x[j].re, H->wint[i][j], b);
           const uint64_t int_i = shoup2(x[j].imag, H->wint[i][j], b);
-          uint64_t lr, hr, li, hi;
-          umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, hr);
-          umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, hi);
-          acc_r = fma((double)lr, two_m64, acc_r);  // __fma_rn (cudaengine.cu:413-414)
-          acc_i = fma((double)li, two_m64, acc_i);
-          rr = addmod_ref(rr, addmod_ref(int_r, hr, b), b);
-          ri = addmod_ref(ri, addmod_ref(int_i, hi, b), b);
+          rr = addmod_ref(rr, addmod_ref(int_r, hr[j], b), b);
+          ri = addmod_ref(ri, addmod_ref(int_i, hi[j], b), b);
         }
       }
-      rr = addmod_ref(rr, (uint64_t)llrint(acc_r), b);
-      ri = addmod_ref(ri, (uint64_t)llrint(acc_i), b);
+      rr = addmod_ref(rr, round_r, b);
+      ri = addmod_ref(ri, round_i, b);
       const cp_gi y = ab[(size_t)i * bs + c];
       yr[i] = addmod_ref(rr, shoup2(y.re, H->lam[i], b), b);
       yi[i] = addmod_ref(ri, shoup2(y.imag, H->lam[i], b), b);

@@ -15,7 +15,7 @@
  *   - or_dgt_forward/inverse   : O(N^2) closed-form DFT over Z_p[i] (Python),
  *                                inverse o forward = id, DGT(const) = const.
  *   - or_polymul               : schoolbook negacyclic product (Python ints)
- *                                and the committed config-1 golden vector.
+ *                                and the committed n = 2048, k = 2 golden vector.
  *   - or_basis_ext_q_to_b /
  *     or_basis_ext_b_to_q      : exact CRT value x mod b_i (Python ints) away
  *                                from the 1e-9 snap window; i^2 mod t KAT.

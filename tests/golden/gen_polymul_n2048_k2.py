@@ -1,10 +1,10 @@
-"""Writes tests/golden/cfg1_polymul.npz: the n=2048, k=2 negacyclic product
+"""Writes tests/golden/polymul_n2048_k2.npz: the n=2048, k=2 negacyclic product
 (the reference's (q120, n=2048) test point, test_bfv_mode.cu:1492-1508).
 
 Inputs come from the Philox uniform sampler (seed 2048, streams 0 and 1).
 The output is computed with the oracle's DGT path and accepted only if it
 equals the schoolbook negacyclic product residue by residue.
-Run from the repository root:  python tests/golden/gen_cfg1.py
+Run from the repository root:  python tests/golden/gen_polymul_n2048_k2.py
 """
 import os
 import sys
@@ -25,7 +25,7 @@ def main():
     c = O.unfold(O.polymul_batch(P, O.fold(a, P.Ng)[None], O.fold(b, P.Ng)[None])[0])
     for r, p in enumerate(P.q):
         assert np.array_equal(c[r], O.schoolbook(a[r], b[r], p))
-    np.savez_compressed(os.path.join(REPO, "tests", "golden", "cfg1_polymul.npz"),
+    np.savez_compressed(os.path.join(REPO, "tests", "golden", "polymul_n2048_k2.npz"),
                         seed=np.uint64(SEED), a=a, b=b, c=c)
     print("ok")
 

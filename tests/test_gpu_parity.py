@@ -67,9 +67,9 @@ def test_polymul_fused_and_plain_match_oracle(gpu, n, k, batch):
     assert np.array_equal(cp.to_host(out2), want), "plain"
 
 
-def test_config1_golden_fixture(gpu):
+def test_polymul_n2048_golden_fixture(gpu):
     import os
-    g = np.load(os.path.join(O.REPO, "tests", "golden", "cfg1_polymul.npz"))
+    g = np.load(os.path.join(O.REPO, "tests", "golden", "polymul_n2048_k2.npz"))
     e, ctx = engine(2, 3, 2048, 256)
     a = cp.to_device(O.fold(g["a"], 1024)[None])
     b = cp.to_device(O.fold(g["b"], 1024)[None])
@@ -244,9 +244,10 @@ def test_large_batch_properties(gpu):
         assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
 
 
-@pytest.mark.parametrize("k,n,batch", [(2, 2048, 3), (4, 16384, 2)])
+@pytest.mark.parametrize("k,n,batch", [(1, 64, 3), (2, 2048, 3), (3, 4096, 2), (5, 8192, 1),
+                                         (4, 16384, 2), (2, 32768, 1)])
 def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
-    """Config-2 shape (n = 2^14, k = 4, kl = 5, t = 65537): tensor product + HPS
+    """n = 2^14, k = 4, kl = 5, t = 65537 (and smaller rings): tensor product + HPS
     rescale of ciphertext pairs, bit-exact against the oracle composition."""
     e, ctx = engine(k, k + 1, n, 65537)
     P = O.Params(k, k + 1, n, 65537)
@@ -261,7 +262,7 @@ def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
 
 @pytest.mark.parametrize("k,n,batch", [(2, 2048, 5), (4, 16384, 3), (3, 32768, 2)])
 def test_sample_muladd_matches_oracle(gpu, k, n, batch):
-    """Config-4 shape (n = 2^14, k = 4): a * s + e with s ternary and e discrete
+    """n = 2^14, k = 4 (and other rings): a * s + e with s ternary and e discrete
     Gaussian drawn on the device, bit-exact against the oracle on the same
     Philox counters."""
     e, ctx = engine(k, k + 1, n)

@@ -344,12 +344,12 @@ def test_schoolbook_matches_python_bigints():
     assert [int(x) for x in got] == [w % p for w in want]
 
 
-def test_config1_golden_vector():
+def test_polymul_n2048_golden_vector():
     """n=2048, k=2 product against the committed fixture (inputs from the
     Philox uniform sampler, seed 2048); the fixture itself was checked
     against the schoolbook product when it was generated."""
     import os
-    path = os.path.join(O.REPO, "tests", "golden", "cfg1_polymul.npz")
+    path = os.path.join(O.REPO, "tests", "golden", "polymul_n2048_k2.npz")
     g = np.load(path)
     P = O.Params(2, 3, 2048, 256)
     a = O.sample_uniform(P.q, P.n, int(g["seed"]), 0)
