@@ -24,6 +24,17 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+# 32-bit VALU lane operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_LANE_OPS_PEAK = 256 * 4 * 16 * 2.4e9
+
+
+def modmuls_per_product(n, k):
+    """63-bit modular products per negacyclic product in the reference's
+    algorithm (two forward and one inverse DGT of N_g = n/2 slots, each
+    N_g log2 N_g butterfly products plus 5 N_g twist/scale products, and 3 N_g
+    for the Gaussian slot-wise product), per residue, times k."""
+    ng = n // 2
+    return k * (3 * (ng * (ng.bit_length() - 1) + 5 * ng) + 3 * ng)
 
 
 def bench_edges(n=16384, k=4, reps=100):
@@ -350,14 +361,25 @@ def main():
 
     # HBM bytes per launch from the committed rocprofv3 PMC summary of this
     # same configuration (FETCH_SIZE / WRITE_SIZE scaled by a copy of known size)
-    traffic = None
+    # and its VALU instruction count (SQ_INSTS_VALU, wave instructions)
+    traffic, valu_insts = None, None
     try:
         with open(os.path.join(REPO, "profiles", "polymul_fused_n16384_k4.json")) as f:
             pm = json.load(f)
         if (pm["n"], pm["k"], pm["batch"]) == (n, k, B):
             traffic = pm["hbm_bytes_per_launch"]
+            valu_insts = pm.get("sq_insts_valu_per_launch")
     except (OSError, ValueError, KeyError):
         pass
+    # the bound of this kernel is integer instruction issue, not HBM: report
+    # the modular-product rate and, from the profile's instruction count, the
+    # fraction of the VALU issue peak the measured launch time implies
+    mm = modmuls_per_product(n, k)
+    integer = {"modmuls_per_product": mm, "modmuls_per_s": value * mm,
+               "valu_lane_ops_peak": VALU_LANE_OPS_PEAK,
+               "valu_insts_per_launch": valu_insts,
+               "valu_issue_frac": (valu_insts * 64 / (avg_launch_ms * 1e-3) / VALU_LANE_OPS_PEAK
+                                   if valu_insts else None)}
 
     cpu = None
     if not args.no_cpu:
@@ -397,6 +419,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_polymul_fused<13,1>", "avg_launch_ms": avg_launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "integer_issue": integer,
         "cpu_baseline": cpu,
         "hom_mul": hm,
         "sample_muladd": enc,
