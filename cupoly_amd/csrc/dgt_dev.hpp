@@ -309,6 +309,27 @@ __device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
+// ---- Exchange synchronisation ------------------------------------------------
+// An exchange between full radix-8 layouts at local stages ls0 and ls0 + 3
+// is wave-local when the earlier layout's unit stride 2^lst is at most 64
+// (lst = LOGM - ls0 - 3 <= 6): each group of 2^lst threads then owns one
+// block of 8 * 2^lst consecutive slots, so the 64 threads of a wave own the
+// same 512 slots (or the same whole rows, when a row has 64 threads or fewer)
+// in both layouts, and read back only what their own wave wrote.  Such an
+// exchange needs no workgroup barrier between its stores and its loads: LDS
+// instructions of one wave execute in program order, and wave_sync() keeps
+// the compiler from moving them across.  The barrier before its stores (no
+// wave may still be reading the region it overwrites) can also go when the
+// previous exchange was wave-local under a map with the same padding ratio
+// (1/8 or none), because every wave then read, and now writes, only its own
+// region.  Everywhere else both barriers stay.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ bool wave_local(int lst) { return lst <= 6; }
+
 // Pass plan over local stages [LSB, LSE): an optional partial pass of
 // R0 = NS mod 3 stages first, then NF full radix-8 passes.  The full passes
 // but the last run in a loop that is not unrolled (so the compiler cannot
@@ -349,15 +370,19 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, PP::LSF, MP);
   }
+  bool prev_local = false, prev_pad = false;  // previous exchange of this loop
 #pragma nounroll
   for (int i = 0; i < PP::NF - 1; i++) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
+    const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
     dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
-    __syncthreads();
+    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0, m);
-    __syncthreads();
+    if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
+    prev_local = local;
+    prev_pad = pad;
   }
   if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 }
@@ -371,15 +396,19 @@ __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
+  bool prev_local = false, prev_pad = false;
 #pragma nounroll
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
-    __syncthreads();
+    const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
+    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
-    __syncthreads();
+    if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
     dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
+    prev_local = local;
+    prev_pad = pad;
   }
   if constexpr (PP::R0 > 0) {
     if constexpr (PP::NF > 0) {

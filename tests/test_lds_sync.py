@@ -1,0 +1,165 @@
+"""LDS exchange synchronisation of the DGT passes, checked on the CPU.
+
+dif_passes / dit_passes (cupoly_amd/csrc/dgt_dev.hpp) skip the workgroup
+barriers of an exchange that is wave-local (each wave reads back only the
+slots it wrote) and, when the previous exchange was also wave-local under a
+map with the same padding ratio, the barrier before its stores.  This test
+replays the LDS addresses those functions compute (the same pass layouts,
+padding maps and skip rules) for every thread of every workgroup shape the
+kernels launch, and requires that any two accesses to one LDS slot from
+different waves, at least one of them a store, are separated by a workgroup
+barrier.  A planted mistake (a cross-wave exchange treated as wave-local)
+must be reported.
+"""
+import bisect
+import collections
+
+import pytest
+
+EPT, LOG_EPT = 8, 3
+
+
+def lds_map_for(lst_lo):
+    return (1, 3) if lst_lo == 0 else (8, 6) if lst_lo == 3 else (0, 0)
+
+
+def lds_row(L):
+    return (1 << L) + ((1 << L) >> 3)
+
+
+def pass_index(L, LS0, R, t, i):
+    M = 1 << L; T = M // EPT; U = 1 << R; ST = M >> (LS0 + R)
+    ju, q = i // U, i % U
+    u = t + ju * T
+    return (u // ST) * (U * ST) + (u % ST) + q * ST
+
+
+def rt_addrs(L, t, ls0, m):
+    A, S = m
+    lst = L - ls0 - LOG_EPT
+    b = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1))
+    pb = b + A * (b >> S)
+    d = (1 << lst) + ((A << (lst - S)) if lst >= S else 0)
+    return [pb + i * d for i in range(EPT)]
+
+
+def fixed_addrs(L, LS0, R, m, t):
+    A, S = m
+    out = []
+    for i in range(EPT):
+        l = pass_index(L, LS0, R, t, i)
+        out.append(l + A * (l >> S))
+    return out
+
+
+def plan(LSB, LSE):
+    NS = LSE - LSB; R0 = NS % LOG_EPT; NF = NS // LOG_EPT
+    return R0, NF, LSB + R0
+
+
+def dif_ops(L, LSB, LSE, local_max=6):
+    R0, NF, LSF = plan(LSB, LSE)
+    MP = lds_map_for(L - LSF - LOG_EPT)
+    ops = []
+    if R0 > 0:
+        if NF == 0:
+            return ops
+        ops += [("sync",), ("st", lambda t: fixed_addrs(L, LSB, R0, MP, t)), ("sync",),
+                ("ld", lambda t, ls=LSF: rt_addrs(L, t, ls, MP))]
+    prev_local = prev_pad = False
+    for i in range(NF - 1):
+        ls0 = LSF + LOG_EPT * i
+        m = lds_map_for(L - ls0 - 2 * LOG_EPT)
+        local = (L - ls0 - LOG_EPT) <= local_max; pad = m[0] != 0
+        ops.append(("wave",) if (local and prev_local and pad == prev_pad) else ("sync",))
+        ops.append(("st", lambda t, ls=ls0, mm=m: rt_addrs(L, t, ls, mm)))
+        ops.append(("wave",) if local else ("sync",))
+        ops.append(("ld", lambda t, ls=ls0 + LOG_EPT, mm=m: rt_addrs(L, t, ls, mm)))
+        prev_local, prev_pad = local, pad
+    return ops
+
+
+def dit_ops(L, LSB, LSE, local_max=6):
+    R0, NF, LSF = plan(LSB, LSE)
+    MP = lds_map_for(L - LSF - LOG_EPT)
+    ops = []
+    prev_local = prev_pad = False
+    for i in range(NF - 2, -1, -1):
+        ls0 = LSF + LOG_EPT * i
+        m = lds_map_for(L - ls0 - 2 * LOG_EPT)
+        local = (L - ls0 - LOG_EPT) <= local_max; pad = m[0] != 0
+        ops.append(("wave",) if (local and prev_local and pad == prev_pad) else ("sync",))
+        ops.append(("st", lambda t, ls=ls0 + LOG_EPT, mm=m: rt_addrs(L, t, ls, mm)))
+        ops.append(("wave",) if local else ("sync",))
+        ops.append(("ld", lambda t, ls=ls0, mm=m: rt_addrs(L, t, ls, mm)))
+        prev_local, prev_pad = local, pad
+    if R0 > 0 and NF > 0:
+        ops += [("sync",), ("st", lambda t: rt_addrs(L, t, LSF, MP)), ("sync",),
+                ("ld", lambda t: fixed_addrs(L, LSB, R0, MP, t))]
+    return ops
+
+
+def check(L, LSB, LSE, rows, seqs):
+    T = (1 << L) // EPT
+    nthreads = T * rows
+    ops = [("sync",)]
+    for s in seqs:
+        ops += s + [("sync",)]
+    acc = collections.defaultdict(list)  # addr -> [(op index, wave, is_store)]
+    span = lds_row(L)
+    for k, op in enumerate(ops):
+        if op[0] not in ("st", "ld"):
+            continue
+        for tid in range(nthreads):
+            row, t = tid // T, tid % T
+            for a in op[1](t):
+                assert 0 <= a < span, (L, a)
+                acc[row * span + a].append((k, tid // 64, op[0] == "st"))
+    syncs = [k for k, op in enumerate(ops) if op[0] == "sync"]
+    def separated(k1, k2):
+        lo, hi = min(k1, k2), max(k1, k2)
+        j = bisect.bisect_right(syncs, lo)
+        return j < len(syncs) and syncs[j] < hi
+    bad = 0
+    for a, lst in acc.items():
+        # per op, group waves; compare consecutive relevant accesses
+        for x in range(len(lst)):
+            for y in range(x + 1, len(lst)):
+                k1, w1, s1 = lst[x]; k2, w2, s2 = lst[y]
+                if w1 == w2 or not (s1 or s2):
+                    continue
+                if k1 == k2 or not separated(k1, k2):
+                    bad += 1
+    nwave = sum(1 for op in ops if op[0] == "wave")
+    return bad, nwave
+
+
+def rows_for(L):
+    T = (1 << L) // EPT
+    return 1 if T >= 256 else 256 // T
+
+
+CASES = [(L, 0, L, rows_for(L)) for L in range(4, 14)]        # single, fused, a*s+e, tensor
+CASES += [(L, 0, L, 8192 >> L) for L in range(10, 14)]         # uneven tiles, 1024 threads
+CASES += [(13, 0, 7, 1), (13, 0, 8, 1), (7, 0, 7, 64)]         # column and block kernels
+
+
+@pytest.mark.parametrize("L,lsb,lse,rows", CASES)
+
+
+def test_exchanges_are_race_free(L, lsb, lse, rows):
+    seq = [dif_ops(L, lsb, lse), dif_ops(L, lsb, lse), dit_ops(L, lsb, lse)]
+    bad, _ = check(L, lsb, lse, rows, seq)
+    assert bad == 0
+
+
+def test_wave_local_exchanges_exist_at_n16384():
+    # the two exchanges closest to the last stages skip barriers at N_g = 8192
+    _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13), dit_ops(13, 0, 13)])
+    assert nwave == 6
+
+
+def test_planted_cross_wave_skip_is_reported():
+    seq = [dif_ops(13, 0, 13, local_max=9), dit_ops(13, 0, 13, local_max=9)]
+    bad, _ = check(13, 0, 13, 1, seq)
+    assert bad > 0
