@@ -194,6 +194,10 @@ def main():
     ap.add_argument("--enc-batch", type=int, default=1024,
                     help="instances per batch of the fused a*s+e sampling step")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-lanes", type=int, default=3,
+                    help="serving: contexts (streams) per ring shape")
+    ap.add_argument("--serve-inflight", type=int, default=2,
+                    help="serving: batches queued on the device per lane")
     ap.add_argument("--serve-rate", type=float, default=20000.0,
                     help="offered load of the serving run, requests/s (Poisson)")
     ap.add_argument("--uneven-count", type=int, default=4096,
@@ -346,13 +350,16 @@ def main():
     serve = None
     if args.serve_requests > 0:
         from cupoly_amd import serving
-        shapes = [serving.Shape(8192, 3), serving.Shape(32768, 8)]
+        shapes = [serving.Shape(8192, 3, lanes=args.serve_lanes),
+                  serving.Shape(32768, 8, lanes=args.serve_lanes)]
         reqs = serving.make_requests(shapes, [0.7, 0.3], args.serve_requests, args.serve_rate,
                                      seed=2024)
         srv = serving.Server(shapes, hbm_budget=32 << 30,
-                             max_batch={shapes[0]: 64, shapes[1]: 16})
+                             max_batch={shapes[0]: 64, shapes[1]: 16},
+                             max_inflight=args.serve_inflight)
         serve = srv.run(reqs)
-        serve.update({"offered_req_per_s": args.serve_rate,
+        serve.update({"offered_req_per_s": args.serve_rate, "lanes_per_shape": args.serve_lanes,
+                      "inflight_per_lane": args.serve_inflight,
                       "mix": "70% n=8192 k=3 / 30% n=32768 k=8, t=65537, tensor + HPS rescale",
                       "hbm_budget_bytes": 32 << 30})
         del reqs, srv, shapes

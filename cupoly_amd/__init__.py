@@ -69,6 +69,7 @@ SIGNATURES = {
     "cupoly_polymul_plain": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_reserve": ([_vp, _int], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
     "cupoly_crt": ([_vp, _vp, _vp, _int, _int, _int], _int),
     "cupoly_icrt": ([_vp, _vp, _vp], _int),
@@ -366,6 +367,19 @@ def hom_mul_gather(ctx, outs, ct1s, ct2s):
     p1 = arr(*[_ptr(t).value for t in ct1s])
     p2 = arr(*[_ptr(t).value for t in ct2s])
     _check(lib().cupoly_hom_mul_gather(ctx.handle, po, p1, p2, b))
+
+
+def hom_mul_reserve(ctx, batch):
+    """Grow ctx's hom-mul workspace and pointer tables for `batch` pairs, so
+    later hom_mul_batch / hom_mul_gather calls of up to `batch` pairs on ctx
+    allocate no device or pinned memory."""
+    _check(lib().cupoly_hom_mul_reserve(ctx.handle, int(batch)))
+
+
+def hom_mul_workspace_bytes(engine, batch):
+    """Device bytes hom_mul_reserve(ctx, batch) holds for engine's shape:
+    7 (k + kl) residue polynomials of N Gaussian slots per pair."""
+    return batch * 7 * (engine.k + engine.kl) * engine.N * 16
 
 
 def sample_muladd_batch(ctx, out, a, seed, stream0):

@@ -265,13 +265,43 @@ int cp_workspace(cupoly_ctx *c, size_t bytes, void **out) {
   return CUPOLY_OK;
 }
 
+// Device pointer table and pinned staging ring of the gathered form.
+static int grow_ptr_tables(cupoly_ctx *c, size_t need) {
+  if (c->ptr_bytes < need) {
+    if (c->d_ptrs) {
+      CP_HIP(hipStreamSynchronize(c->stream));
+      CP_HIP(hipFree(c->d_ptrs));
+      c->d_ptrs = nullptr;
+      c->ptr_bytes = 0;
+    }
+    CP_HIP(hipMalloc((void **)&c->d_ptrs, need));
+    c->ptr_bytes = need;
+  }
+  if (c->ring_slot_bytes < need) {
+    if (c->h_ring) {
+      CP_HIP(hipStreamSynchronize(c->stream));
+      CP_HIP(hipHostFree(c->h_ring));
+      c->h_ring = nullptr;
+    }
+    CP_HIP(hipHostMalloc(&c->h_ring, need * cupoly_ctx::kRing, hipHostMallocDefault));
+    c->ring_slot_bytes = need;
+    for (int i = 0; i < cupoly_ctx::kRing; i++) c->ring_used[i] = false;
+  }
+  return CUPOLY_OK;
+}
+
+// (k + kl)-residue scratch of the pipeline: 4 extended inputs and 3 products per pair.
+static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
+  return (size_t)batch * 7 * (e->k + e->kl) * e->ng * sizeof(cp_gi);
+}
+
 static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
   cupoly_engine *e = c->e;
   const int k = e->k, kl = e->kl, nres = k + kl, ng = e->ng;
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
-  const size_t xs = (size_t)batch * 4 * nres * ng, ys = (size_t)batch * 3 * nres * ng;
+  const size_t xs = (size_t)batch * 4 * nres * ng;
   void *ws;
-  int rc = cp_workspace(c, (xs + ys) * sizeof(cp_gi), &ws);
+  int rc = cp_workspace(c, hom_mul_ws_bytes(e, batch), &ws);
   if (rc) return rc;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
@@ -315,26 +345,8 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   // pointer tables to the device on the context's stream, from a pinned
   // staging slot that is not rewritten until its previous copy has finished
   const size_t need = (size_t)batch * 3 * sizeof(void *);
-  if (c->ptr_bytes < need) {
-    if (c->d_ptrs) {
-      CP_HIP(hipStreamSynchronize(c->stream));
-      CP_HIP(hipFree(c->d_ptrs));
-      c->d_ptrs = nullptr;
-      c->ptr_bytes = 0;
-    }
-    CP_HIP(hipMalloc((void **)&c->d_ptrs, need));
-    c->ptr_bytes = need;
-  }
-  if (c->ring_slot_bytes < need) {
-    if (c->h_ring) {
-      CP_HIP(hipStreamSynchronize(c->stream));
-      CP_HIP(hipHostFree(c->h_ring));
-      c->h_ring = nullptr;
-    }
-    CP_HIP(hipHostMalloc(&c->h_ring, need * cupoly_ctx::kRing, hipHostMallocDefault));
-    c->ring_slot_bytes = need;
-    for (int i = 0; i < cupoly_ctx::kRing; i++) c->ring_used[i] = false;
-  }
+  int rc = grow_ptr_tables(c, need);
+  if (rc) return rc;
   const int slot = c->ring_next;
   c->ring_next = (slot + 1) % cupoly_ctx::kRing;
   if (!c->ring_ev[slot]) CP_HIP(hipEventCreateWithFlags(&c->ring_ev[slot], hipEventDisableTiming));
@@ -351,4 +363,14 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   cp_gi **d = (cp_gi **)c->d_ptrs;
   cp_cts o{nullptr, 0, d}, a{nullptr, 0, d + batch}, b{nullptr, 0, d + 2 * batch};
   return hom_mul_run(c, o, a, b, batch);
+}
+
+extern "C" int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch) {
+  if (!c || batch < 0 || batch > 65535 / 4)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_reserve: bad argument");
+  if (batch == 0) return CUPOLY_OK;
+  void *ws;
+  int rc = cp_workspace(c, hom_mul_ws_bytes(c->e, batch), &ws);
+  if (rc) return rc;
+  return grow_ptr_tables(c, (size_t)batch * 3 * sizeof(void *));
 }

@@ -2,14 +2,19 @@
 
 Requests (a pair of ciphertexts of one ring shape) arrive over time.  The loop
   * keeps one CUDAEngine per ring shape resident (twiddle, twist and HPS
-    tables stay in HBM for the whole run) and one context + stream per shape,
-    so batches of different shapes overlap on the device;
+    tables stay in HBM for the whole run) and `lanes` contexts per shape, each
+    with its own stream and workspace, so batches of one shape and of
+    different shapes overlap on the device (a batch of one small request
+    fills only a few dozen CUs);
   * admits arrived requests while their HBM footprint (output buffers plus
     their share of the pipeline workspace) fits the budget; the rest wait;
-  * whenever a shape has fewer than `max_inflight` batches on the device, it
-    launches every admitted request of that shape (up to `max_batch`) as one
-    gathered batch (cupoly_hom_mul_gather), so requests that arrive while
-    others are on the GPU join the next batch instead of waiting for a drain;
+  * sizes every lane's workspace for its largest batch before the clock
+    starts (cupoly_hom_mul_reserve), so no request waits on an allocation;
+  * whenever a lane of a shape has fewer than `max_inflight` batches on the
+    device, it launches every admitted request of that shape (up to
+    `max_batch`) as one gathered batch (cupoly_hom_mul_gather) on that lane,
+    so requests that arrive while others are on the GPU join the next batch
+    instead of waiting for a drain;
   * timestamps completions with device events, and reports throughput and
     p50 / p99 latency from arrival to result-ready.
 
@@ -25,12 +30,13 @@ import cupoly_amd as cp
 
 
 class Shape:
-    def __init__(self, n, k, kl=None, t=65537):
+    def __init__(self, n, k, kl=None, t=65537, lanes=2):
         import torch
         self.n, self.k, self.kl, self.t = n, k, kl or k + 1, t
         self.engine = cp.CUDAEngine(self.k, self.kl, n, t)
-        self.stream = torch.cuda.Stream()  # default priority
-        self.ctx = cp.Context(self.engine, stream=self.stream)
+        self.streams = [torch.cuda.Stream() for _ in range(lanes)]  # default priority
+        self.ctxs = [cp.Context(self.engine, stream=st) for st in self.streams]
+        self.stream, self.ctx = self.streams[0], self.ctxs[0]
         ng = n // 2
         self.ct_shape = (2, self.k, ng, 2)
         self.out_shape = (3, self.k, ng, 2)
@@ -80,14 +86,21 @@ class Server:
     def __init__(self, shapes, hbm_budget, max_batch=None, max_inflight=2):
         self.shapes = shapes
         self.hbm_budget = hbm_budget
-        self.max_batch = max_batch or {sh: 64 for sh in shapes}
+        max_batch = max_batch or {sh: 64 for sh in shapes}
+        # a batch never holds more requests than the budget admits at once
+        self.max_batch = {sh: max(1, min(max_batch[sh], hbm_budget // sh.footprint))
+                          for sh in shapes}
         self.max_inflight = max_inflight
+        for sh in shapes:
+            for ctx in sh.ctxs:
+                cp.hom_mul_reserve(ctx, self.max_batch[sh])
 
     def run(self, requests):
         import torch
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queues = {sh: collections.deque() for sh in self.shapes}
-        inflight = {sh: collections.deque() for sh in self.shapes}  # (event, [requests])
+        # per shape and lane: batches on the device, oldest first (event, [requests])
+        inflight = {sh: [collections.deque() for _ in sh.ctxs] for sh in self.shapes}
         used = 0
         peak_used = 0
         batches = 0
@@ -105,30 +118,35 @@ class Server:
                 used += r.shape.footprint
                 queues[r.shape].append(r)
             peak_used = max(peak_used, used)
-            # launch: every shape with room on the device takes its whole queue
+            # launch: every shape with a lane that has room takes its whole
+            # queue (up to max_batch) onto the least-loaded such lane
             for sh in self.shapes:
                 q = queues[sh]
-                if q and len(inflight[sh]) < self.max_inflight:
+                while q:
+                    lane = min(range(len(sh.ctxs)), key=lambda i: len(inflight[sh][i]))
+                    if len(inflight[sh][lane]) >= self.max_inflight:
+                        break
                     take = [q.popleft() for _ in range(min(len(q), self.max_batch[sh]))]
-                    cp.hom_mul_gather(sh.ctx, [r.out for r in take], [r.ct1 for r in take],
-                                      [r.ct2 for r in take])
+                    cp.hom_mul_gather(sh.ctxs[lane], [r.out for r in take],
+                                      [r.ct1 for r in take], [r.ct2 for r in take])
                     ev = torch.cuda.Event(enable_timing=True)
-                    ev.record(sh.stream)
+                    ev.record(sh.streams[lane])
                     for r in take:
                         r.batch = batches
                     batches += 1
-                    inflight[sh].append((ev, take))
+                    inflight[sh][lane].append((ev, take))
             # completions (batches of one stream finish in order)
             progressed = False
             for sh in self.shapes:
-                while inflight[sh] and inflight[sh][0][0].query():
-                    ev, take = inflight[sh].popleft()
-                    ms = start.elapsed_time(ev)
-                    for r in take:
-                        r.done_ms = ms
-                        used -= sh.footprint
-                        done.append(r)
-                    progressed = True
+                for fl in inflight[sh]:
+                    while fl and fl[0][0].query():
+                        ev, take = fl.popleft()
+                        ms = start.elapsed_time(ev)
+                        for r in take:
+                            r.done_ms = ms
+                            used -= sh.footprint
+                            done.append(r)
+                        progressed = True
             if not progressed and not any(queues[s] for s in self.shapes):
                 if pending and pending[0].arrival > time.perf_counter() - t0:
                     time.sleep(min(50e-6, max(0.0, pending[0].arrival - (time.perf_counter() - t0))))

@@ -201,6 +201,12 @@ int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, co
  * to the device on the context's stream before the call returns. */
 int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *const *ct1,
                           const cupoly_gi *const *ct2, int batch);
+/* Grows the context's hom-mul workspace and gathered-form pointer tables to
+ * hold `batch` pairs, so later hom_mul_batch / hom_mul_gather calls of up to
+ * that many pairs on this context allocate nothing (no reference counterpart:
+ * the reference allocates its Context scratch once, context.cpp:19-67; a
+ * serving loop calls this before it starts timing requests). */
+int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
 
 /* Encryption-shaped step for a batch: out[i] = a[i] * s_i + e_i in the DGT
  * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian

@@ -35,3 +35,23 @@ def test_admission_respects_hbm_budget(gpu):
     stats = serving.Server(shapes, hbm_budget=budget).run(reqs)
     assert stats["requests"] == 30
     assert stats["peak_admitted_bytes"] <= budget
+
+
+def test_hom_mul_reserve_keeps_results(gpu):
+    shapes = [serving.Shape(2048, 2, lanes=1)]
+    reqs = serving.make_requests(shapes, [1.0], 3, 1e6, seed=7)
+    sh = shapes[0]
+    fresh = cp.Context(sh.engine)
+    cp.hom_mul_reserve(sh.ctx, 8)
+    a = cp.empty((3,) + sh.out_shape)
+    b = cp.empty((3,) + sh.out_shape)
+    ct1 = torch.stack([r.ct1 for r in reqs])
+    ct2 = torch.stack([r.ct2 for r in reqs])
+    cp.hom_mul_batch(sh.ctx, a, ct1, ct2)
+    cp.hom_mul_batch(fresh, b, ct1, ct2)
+    sh.ctx.sync()
+    fresh.sync()
+    assert torch.equal(a, b)
+    with pytest.raises(RuntimeError):
+        cp.hom_mul_reserve(sh.ctx, -1)
+    fresh.destroy()
