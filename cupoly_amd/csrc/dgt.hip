@@ -308,6 +308,32 @@ __global__ void __launch_bounds__(1024)
   for (int i = 0; i < EPT; i++) pa[map(first_index<LOGB, 0, LOGB>(t, i))] = xa[i];
 }
 
+// Product for N = 16384 slots (n = 2^15): one 1024-thread workgroup per
+// residue product (polymul_16k_unit), persistent over the batch; each
+// workgroup owns 3 * 8192 park slots.
+__global__ void __launch_bounds__(1024)
+    k_polymul_16k(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
+                  const cp_gi *__restrict__ b, int nunits, int nres,
+                  const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                  const uint64_t *__restrict__ twi_all, const uint64_t *__restrict__ tsf_all,
+                  const uint64_t *__restrict__ tsi_all, cp_gi *__restrict__ park_all) {
+  constexpr int N = 16384;
+  extern __shared__ cp_gi lds[];
+  CP_GLOBAL cp_gi *park = (CP_GLOBAL cp_gi *)(park_all + (size_t)blockIdx.x * 3 * (N / 2));
+  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {  // uniform trip count per workgroup
+    const int rid = unit % nres;
+    const cp_prime pr = primes[rid];
+    const size_t off = (size_t)unit * N;
+    polymul_16k_unit((CP_GLOBAL cp_gi *)(out + off), (const CP_GLOBAL cp_gi *)(a + off),
+                     (const CP_GLOBAL cp_gi *)(b + off), park, lds, pr,
+                     (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N),
+                     (const CP_GLOBAL uint64_t *)(twi_all + (size_t)rid * N),
+                     (const CP_GLOBAL uint64_t *)(tsf_all + (size_t)rid * N * 6),
+                     (const CP_GLOBAL uint64_t *)(tsi_all + (size_t)rid * N * 6));
+    __syncthreads();
+  }
+}
+
 // -------------------------------------------------------------------------
 // Host side
 // -------------------------------------------------------------------------
@@ -336,6 +362,8 @@ static int setup_size() {
     if ((rc = set_lds(k_dgt_blocks<LOGN, true>, lds))) return rc;
     if ((rc = set_lds(k_dgt_blocks<LOGN, false>, lds))) return rc;
     if ((rc = set_lds(k_polymul_blocks<LOGN>, lds))) return rc;
+    if constexpr (LOGN == 14)
+      if ((rc = set_lds(k_polymul_16k, lds))) return rc;
   }
   return CUPOLY_OK;
 }
@@ -423,6 +451,17 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
                        c->stream, out, a, b, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
                        e->d_twist_fwd, e->d_twist_inv);
+    CP_LAUNCH_CHECK();
+  } else if constexpr (LOGN == 14) {
+    // persistent: one workgroup per CU, each with its own 384 KiB park area
+    const int grid = nunits < e->ncu ? nunits : e->ncu;
+    void *ws;
+    int rc = cp_workspace(c, (size_t)grid * 3 * (1 << (LOGN - 1)) * sizeof(cp_gi), &ws);
+    if (rc) return rc;
+    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
+    hipLaunchKernelGGL(k_polymul_16k, dim3(grid), dim3(1024), lds, c->stream, out, a, b, nunits,
+                       nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv, e->d_twist_fwd,
+                       e->d_twist_inv, (cp_gi *)ws);
     CP_LAUNCH_CHECK();
   } else {
     // out <- a, scratch <- b (caller inputs are not modified); transforms in place

@@ -457,3 +457,109 @@ template <int LOGN>
 constexpr int rows_for() {
   return ((1 << LOGN) / EPT) >= 256 ? 1 : 256 / ((1 << LOGN) / EPT);
 }
+
+// ---- n = 2^15: one residue product of N = 16384 slots in one workgroup -----
+// Forward stage 0 pairs slot g with g + 8192 (twiddle omega^g).  Thread t of a
+// 1024-thread workgroup loads slot g = first_index<13,0,13>(t, i) of both
+// halves, so stage 0 (and, at the end, its inverse) runs in registers;
+// stages 1..13 run on each 8192-slot half in turn through one 144 KiB LDS
+// array (the local transform at stage offset 1).  While one half is in the
+// passes the other waits in `park`, a per-workgroup global area of 3 x 8192
+// slots addressed thread-major (slot i of thread t at i * 1024 + t: every
+// access is coalesced and only the owning thread reads it back), so only
+// 8 slots per thread are live during the passes, as in the 8192-slot
+// kernels.  Park areas: P0 = a's half-0 spectrum, later the half-0 inverse;
+// P1 = a's half 1; P2 = b's half 1.
+__device__ __forceinline__ void park_st(CP_GLOBAL cp_gi *park, int area, int t,
+                                        const cp_gi (&x)[EPT]) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) st_gi(park + (area * EPT + i) * 1024 + t, x[i]);
+}
+__device__ __forceinline__ void park_ld(const CP_GLOBAL cp_gi *park, int area, int t,
+                                        cp_gi (&x)[EPT]) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = ld_gi(park + (area * EPT + i) * 1024 + t);
+}
+
+// twist + forward stage 0 of src; half 0 -> x0, half 1 -> x1
+__device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
+                                           const CP_GLOBAL cp_gi *src, int t,
+                                           const CP_GLOBAL uint64_t *twf,
+                                           const CP_GLOBAL uint64_t *tsf, uint64_t p) {
+  constexpr int H = 8192;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<13, 0, 13>(t, i);
+    cp_gi x = ld_gi(src + g), y = ld_gi(src + g + H);
+    twist(x, tsf, g, p);
+    twist(y, tsf, g + H, p);
+    uint64_t w, wp;
+    ld_tw(twf, (unsigned)g, w, wp);
+    x0[i].re = cp_red2(x.re + y.re, p);
+    x0[i].imag = cp_red2(x.imag + y.imag, p);
+    x1[i].re = cp_shoup(x.re + p - y.re, w, wp, p);
+    x1[i].imag = cp_shoup(x.imag + p - y.imag, w, wp, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// `park` must hold 3 * 8192 slots for this workgroup.
+__device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
+                                                 const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
+                                                 cp_gi *lds, const cp_prime &pr,
+                                                 const CP_GLOBAL uint64_t *twf,
+                                                 const CP_GLOBAL uint64_t *twi,
+                                                 const CP_GLOBAL uint64_t *tsf,
+                                                 const CP_GLOBAL uint64_t *tsi) {
+  constexpr int H = 8192;
+  const int t = (int)threadIdx.x;
+  const uint64_t p = pr.p;
+  cp_gi x[EPT], y[EPT];
+  // a: spectrum halves to P0, P1
+  stage0_16k(x, y, a, t, twf, tsf, p);
+  park_st(park, 1, t, y);
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
+  park_st(park, 0, t, x);
+  park_ld(park, 1, t, y);
+  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H});
+  park_st(park, 1, t, y);
+  // b, half 0: spectrum, product with P0, inverse stages 13..1, back to P0
+  int tb = t;
+  asm volatile("" : "+v"(tb));
+  stage0_16k(x, y, b, tb, twf, tsf, p);
+  park_st(park, 2, t, y);
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
+  park_ld(park, 0, t, y);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = cp_gimul(x[i], y[i], pr);
+  dit_passes<14, 13, 0, 13, 1>(x, lds, t, twi, p, MapBlock{0});
+  park_st(park, 0, t, x);
+  // b, half 1 with P1
+  park_ld(park, 2, t, y);
+  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H});
+  park_ld(park, 1, t, x);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) y[i] = cp_gimul(y[i], x[i], pr);
+  dit_passes<14, 13, 0, 13, 1>(y, lds, t, twi, p, MapBlock{H});
+  park_ld(park, 0, t, x);
+  // inverse stage 0 (twiddle omega^-g) and untwist (xi^-n / N)
+  int to = t;
+  asm volatile("" : "+v"(to));
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<13, 0, 13>(to, i);
+    uint64_t w, wp;
+    ld_tw(twi, (unsigned)g, w, wp);
+    const uint64_t tr = cp_shoup(y[i].re, w, wp, p), ti = cp_shoup(y[i].imag, w, wp, p);
+    cp_gi s, d;
+    s.re = cp_red2(x[i].re + tr, p);
+    s.imag = cp_red2(x[i].imag + ti, p);
+    d.re = cp_red2(x[i].re + p - tr, p);
+    d.imag = cp_red2(x[i].imag + p - ti, p);
+    twist(s, tsi, g, p);
+    twist(d, tsi, g + H, p);
+    st_gi(c + g, s);
+    st_gi(c + g + H, d);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}

@@ -206,6 +206,8 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
   compute_hps(e->h_hps, e->primes.data(), k, kl, t);
   compute_crt(e->h_crt, e->primes.data(), k, e->nres);
   hipError_t err = hipGetDevice(&e->device);
+  if (err == hipSuccess)
+    err = hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, e->device);
   if (err != hipSuccess) {
     delete e;
     return cp_check_hip(err, "hipGetDevice");

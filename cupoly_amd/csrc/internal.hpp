@@ -47,6 +47,7 @@ struct cupoly_engine {
   int k = 0, kl = 0, n = 0, ng = 0, logng = 0, nres = 0;
   uint64_t t = 0;
   int device = 0;
+  int ncu = 256;  // compute units of `device`
   std::vector<uint64_t> primes;      // Q then B
   cp_prime *d_primes = nullptr;      // [nres]
   // DGT tables, per residue r (all of Q then B):

@@ -10,9 +10,10 @@
 //     by side by a 1024-thread workgroup (forward a, forward b, product,
 //     inverse, as k_polymul_fused);
 //   * n = 2^15 (16384 slots, 256 KiB, more than the LDS): one unit per tile;
-//     the workgroup does the first (last) radix-2 stage as a streaming pass
-//     through a private 512 KiB global scratch and the remaining 13 stages on
-//     each 8192-slot half in LDS.
+//     each thread loads 8 slots of both halves, so the first (last) radix-2
+//     stage runs in registers, and the remaining 13 stages run on each
+//     8192-slot half in turn through LDS while the other half waits in a
+//     private 384 KiB global scratch (polymul_16k_unit).
 // A persistent grid of workgroups pulls tiles from an atomic counter until
 // the list is exhausted; tile order and grid size are tunables picked by
 // cupoly_uneven_autotune on the real batch.  The counter is reset by a small
@@ -62,7 +63,7 @@ struct cupoly_uneven {
   cp_tile *d_tiles = nullptr;
   cp_unit *d_units = nullptr;
   int *d_counter = nullptr;
-  cp_gi *d_scratch = nullptr;  // 2 x 16384 slots per workgroup (n = 2^15 tiles only)
+  cp_gi *d_scratch = nullptr;  // 3 x 8192 slots per workgroup (n = 2^15 tiles only)
   size_t scratch_wgs = 0;
   bool has_16k = false;
   int ntiles = 0;
@@ -159,96 +160,25 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   }
 }
 
-// n = 2^15: 16384 slots; scratch = 2 x 16384 slots private to this workgroup.
+// n = 2^15: 16384 slots (polymul_16k_unit); the workgroup's scratch holds
+// the 3 * 8192 parked slots.
 __device__ __forceinline__ void uneven_16k(const cp_unit *units, int unit0, const uint64_t *twf_c,
                                         const uint64_t *twi_c, const uint64_t *tsf_c,
                                         const uint64_t *tsi_c, const cp_prime *primes,
                                         cp_gi *scratch) {
-  constexpr int LOGN = 14, N = 1 << LOGN, H = N / 2;
-  cp_gi *lds = uneven_lds;
-  const int t = (int)threadIdx.x;
+  constexpr int N = 16384;
   const cp_unit u = ld_unit(units, unit0);
   const cp_prime pr = ld_prime(primes, u.rid);
-  const uint64_t p = pr.p;
-  const CP_GLOBAL uint64_t *twf = glb(twf_c) + (size_t)u.rid * N;
-  const CP_GLOBAL uint64_t *twi = glb(twi_c) + (size_t)u.rid * N;
-  const CP_GLOBAL uint64_t *tsf = glb(tsf_c) + (size_t)u.rid * N * 6;
-  const CP_GLOBAL uint64_t *tsi = glb(tsi_c) + (size_t)u.rid * N * 6;
-  CP_GLOBAL cp_gi *A = glb(scratch), *B = glb(scratch) + N;
-  const CP_GLOBAL cp_gi *ua = glb(u.a), *ub = glb(u.b);
-  CP_GLOBAL cp_gi *uc = glb(u.c);
-  // forward stage 0 (pairs i, i + 8192, twiddle omega^i) after the twist,
-  // streaming from the input into the scratch half-arrays
-  for (int which = 0; which < 2; which++) {
-    const CP_GLOBAL cp_gi *src = which ? ub : ua;
-    CP_GLOBAL cp_gi *dst = which ? B : A;
-    for (int i = t; i < H; i += UNEVEN_THREADS) {
-      cp_gi x = ld_gi(src + i), y = ld_gi(src + i + H);
-      twist(x, tsf, i, p);
-      twist(y, tsf, i + H, p);
-      uint64_t w, wp;
-      ld_tw(twf, i, w, wp);
-      cp_gi s, d;
-      s.re = cp_red2(x.re + y.re, p);
-      s.imag = cp_red2(x.imag + y.imag, p);
-      d.re = cp_shoup(x.re + p - y.re, w, wp, p);
-      d.imag = cp_shoup(x.imag + p - y.imag, w, wp, p);
-      st_gi(dst + i, s);
-      st_gi(dst + i + H, d);
-    }
-  }
-  __syncthreads();
-  using LP = LastPass<13, 0, 13>;
-  // stages 1..13 of a, one half at a time in LDS; spectra back to A
-  for (int h = 0; h < 2; h++) {
-    MapBlock map{h * H};
-    cp_gi x[EPT];
-#pragma unroll
-    for (int i = 0; i < EPT; i++) x[i] = ld_gi(A + h * H + first_index<13, 0, 13>(t, i));
-    dif_passes<LOGN, 13, 0, 13, 1>(x, lds, t, twf, p, map);
-#pragma unroll
-    for (int i = 0; i < EPT; i++) st_gi(A + h * H + pass_index<13, LP::LS0, LP::RL>(t, i), x[i]);
-    __syncthreads();
-  }
-  // b's halves, product with a's spectrum, inverse stages 13..1, back to B
-  for (int h = 0; h < 2; h++) {
-    MapBlock map{h * H};
-    cp_gi x[EPT];
-#pragma unroll
-    for (int i = 0; i < EPT; i++) x[i] = ld_gi(B + h * H + first_index<13, 0, 13>(t, i));
-    dif_passes<LOGN, 13, 0, 13, 1>(x, lds, t, twf, p, map);
-#pragma unroll
-    for (int i = 0; i < EPT; i++)
-      x[i] = cp_gimul(x[i], ld_gi(A + h * H + pass_index<13, LP::LS0, LP::RL>(t, i)), pr);
-    dit_passes<LOGN, 13, 0, 13, 1>(x, lds, t, twi, p, map);
-    __syncthreads();  // every thread has read its B half-h inputs
-#pragma unroll
-    for (int i = 0; i < EPT; i++) st_gi(B + h * H + first_index<13, 0, 13>(t, i), x[i]);
-  }
-  __syncthreads();
-  // inverse stage 0 (twiddle omega^-i) and untwist (xi^-n / N), to the output
-  for (int i = t; i < H; i += UNEVEN_THREADS) {
-    const cp_gi x = ld_gi(B + i), y = ld_gi(B + i + H);
-    uint64_t w, wp;
-    ld_tw(twi, i, w, wp);
-    const uint64_t tr = cp_shoup(y.re, w, wp, p), ti = cp_shoup(y.imag, w, wp, p);
-    cp_gi s, d;
-    s.re = cp_red2(x.re + tr, p);
-    s.imag = cp_red2(x.imag + ti, p);
-    d.re = cp_red2(x.re + p - tr, p);
-    d.imag = cp_red2(x.imag + p - ti, p);
-    twist(s, tsi, i, p);
-    twist(d, tsi, i + H, p);
-    st_gi(uc + i, s);
-    st_gi(uc + i + H, d);
-  }
+  polymul_16k_unit(glb(u.c), glb(u.a), glb(u.b), glb(scratch), uneven_lds, pr,
+                   glb(twf_c) + (size_t)u.rid * N, glb(twi_c) + (size_t)u.rid * N,
+                   glb(tsf_c) + (size_t)u.rid * N * 6, glb(tsi_c) + (size_t)u.rid * N * 6);
 }
 
 __global__ void __launch_bounds__(UNEVEN_THREADS)
     k_uneven(const cp_tile *__restrict__ tiles, const cp_unit *__restrict__ units, int ntiles,
              int *__restrict__ counter, cp_class_tables T, cp_gi *__restrict__ scratch) {
   __shared__ int s_tile;
-  cp_gi *my_scratch = scratch + (size_t)blockIdx.x * 2 * 16384;
+  cp_gi *my_scratch = scratch + (size_t)blockIdx.x * 3 * 8192;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
     __syncthreads();
@@ -292,7 +222,7 @@ static int ensure_scratch(cupoly_uneven *u, int grid) {
     CP_HIP(hipFree(u->d_scratch));
   }
   u->d_scratch = nullptr;
-  CP_HIP(hipMalloc((void **)&u->d_scratch, (size_t)grid * 2 * 16384 * sizeof(cp_gi)));
+  CP_HIP(hipMalloc((void **)&u->d_scratch, (size_t)grid * 3 * 8192 * sizeof(cp_gi)));
   u->scratch_wgs = grid;
   return CUPOLY_OK;
 }
