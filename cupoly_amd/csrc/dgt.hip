@@ -452,9 +452,11 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
                        c->stream, out, a, b, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
                        e->d_twist_fwd, e->d_twist_inv);
     CP_LAUNCH_CHECK();
-  } else if constexpr (LOGN == 14) {
-    // persistent: one workgroup per CU, each with its own 384 KiB park area
-    const int grid = nunits < e->ncu ? nunits : e->ncu;
+  } else if (LOGN == 14 && nunits >= e->ncu) {
+    // persistent: one workgroup per CU, each with its own 384 KiB park area.
+    // Fewer units than CUs keep the split below, which spreads each product
+    // over twice as many workgroups.
+    const int grid = e->ncu;
     void *ws;
     int rc = cp_workspace(c, (size_t)grid * 3 * (1 << (LOGN - 1)) * sizeof(cp_gi), &ws);
     if (rc) return rc;

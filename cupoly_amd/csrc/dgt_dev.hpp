@@ -503,6 +503,32 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
   }
 }
 
+// inverse stage 0 (twiddle omega^-g) and untwist (xi^-n / N) of slot g and
+// g + 8192, written to out
+__device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&x)[EPT],
+                                            const cp_gi (&y)[EPT], int t,
+                                            const CP_GLOBAL uint64_t *twi,
+                                            const CP_GLOBAL uint64_t *tsi, uint64_t p) {
+  constexpr int H = 8192;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<13, 0, 13>(t, i);
+    uint64_t w, wp;
+    ld_tw(twi, (unsigned)g, w, wp);
+    const uint64_t tr = cp_shoup(y[i].re, w, wp, p), ti = cp_shoup(y[i].imag, w, wp, p);
+    cp_gi s, dd;
+    s.re = cp_red2(x[i].re + tr, p);
+    s.imag = cp_red2(x[i].imag + ti, p);
+    dd.re = cp_red2(x[i].re + p - tr, p);
+    dd.imag = cp_red2(x[i].imag + p - ti, p);
+    twist(s, tsi, g, p);
+    twist(dd, tsi, g + H, p);
+    st_gi(out + g, s);
+    st_gi(out + g + H, dd);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // `park` must hold 3 * 8192 slots for this workgroup.
 __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
                                                  const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
@@ -542,24 +568,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   for (int i = 0; i < EPT; i++) y[i] = cp_gimul(y[i], x[i], pr);
   dit_passes<14, 13, 0, 13, 1>(y, lds, t, twi, p, MapBlock{H});
   park_ld(park, 0, t, x);
-  // inverse stage 0 (twiddle omega^-g) and untwist (xi^-n / N)
   int to = t;
   asm volatile("" : "+v"(to));
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    const int g = first_index<13, 0, 13>(to, i);
-    uint64_t w, wp;
-    ld_tw(twi, (unsigned)g, w, wp);
-    const uint64_t tr = cp_shoup(y[i].re, w, wp, p), ti = cp_shoup(y[i].imag, w, wp, p);
-    cp_gi s, d;
-    s.re = cp_red2(x[i].re + tr, p);
-    s.imag = cp_red2(x[i].imag + ti, p);
-    d.re = cp_red2(x[i].re + p - tr, p);
-    d.imag = cp_red2(x[i].imag + p - ti, p);
-    twist(s, tsi, g, p);
-    twist(d, tsi, g + H, p);
-    st_gi(c + g, s);
-    st_gi(c + g + H, d);
-    __builtin_amdgcn_sched_barrier(0);
-  }
+  istage0_16k(c, x, y, to, twi, tsi, p);
 }

@@ -13,8 +13,10 @@
 //   1. k_ext_q_to_qb:   each of the 4 inputs -> QB residues (Q copied, B by the
 //                       HPS extension), one thread per Gaussian slot;
 //   2. forward DGT of the 4*(k+kl) residue polynomials per pair;
-//   3. k_tensor:        slot-wise c0 d0, c0 d1 + c1 d0, c1 d1 in every residue;
-//   4. inverse DGT of the 3*(k+kl) results;
+//   3.+4. k_tensor_idgt: slot-wise c0 d0, c0 d1 + c1 d0, c1 d1 formed as the
+//                       inverse DGT of each of the 3*(k+kl) results loads its
+//                       operands (n <= 2^14; above, k_tensor then the
+//                       two-kernel inverse DGT);
 //   5. k_scale_to_q:    complex scaling t/Q (QB -> B) and B -> Q fused in
 //                       registers, one thread per slot, writing base Q.
 // Scratch lives in a grow-only workspace owned by the context.

@@ -67,6 +67,28 @@ def test_polymul_fused_and_plain_match_oracle(gpu, n, k, batch):
     assert np.array_equal(cp.to_host(out2), want), "plain"
 
 
+def test_polymul_n32768_persistent_matches_split_and_oracle(gpu):
+    """n = 2^15 batches of at least one residue product per CU take the
+    one-workgroup-per-product persistent kernel; smaller ones the two-kernel
+    split.  The same 96 products (288 residue products) run both ways must
+    agree everywhere, and three of them with the oracle."""
+    n, k, batch = 32768, 3, 96
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 77)
+    b = rand_res(P, P.q, (batch,), 78)
+    da, db = cp.to_device(a), cp.to_device(b)
+    whole = cp.empty(a.shape)
+    cp.polymul_batch(ctx, whole, da, db)
+    parts = cp.empty(a.shape)
+    for i in range(0, batch, 16):  # 48 residue products per call
+        cp.polymul_batch(ctx, parts[i:i + 16], da[i:i + 16], db[i:i + 16])
+    h = cp.to_host(whole)
+    assert np.array_equal(h, cp.to_host(parts))
+    for i in (0, 41, 95):
+        assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0]), i
+
+
 def test_polymul_n2048_golden_fixture(gpu):
     import os
     g = np.load(os.path.join(O.REPO, "tests", "golden", "polymul_n2048_k2.npz"))
