@@ -78,28 +78,59 @@ __global__ void __launch_bounds__(256)
 }
 
 // One workgroup: the h coefficients (of n = 2 ng) with the smallest keys
-// become +-1, all others 0; every residue gets the value mod q_r.
+// become +-1, all others 0; every residue gets the value mod q_r.  The h-th
+// smallest key is found by radix selection, 8 bits per pass from the top.
+// Selection stops as soon as every key sharing the resolved top bits with
+// the h-th one is itself selected (its bucket holds exactly the remaining
+// rank, e.g. a bucket of one key): then "top bits <= the resolved prefix"
+// picks the same h keys as a full 64-bit comparison, usually after 2 or 3
+// passes instead of 8.  With KPT > 0 each thread keeps its KPT keys (and
+// signs) in registers, so Philox runs once per coefficient; KPT = 0 (n above
+// 16384) recomputes them in each pass.
 #define FW_THREADS 1024
+template <int KPT>
 __global__ void __launch_bounds__(FW_THREADS)
     k_sample_fixed_weight(cp_gi *__restrict__ out, int ng, int nres, int roff,
                           const cp_prime *__restrict__ primes, uint64_t seed, uint64_t stream,
                           int h) {
   __shared__ unsigned hist[256];
   __shared__ uint64_t s_prefix;
-  __shared__ int s_need;
+  __shared__ int s_need, s_done;
   const int n = 2 * ng, tid = (int)threadIdx.x;
+  uint64_t keys[KPT > 0 ? KPT : 1];
+  uint32_t negs = 0;
+  if constexpr (KPT > 0) {
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int j = tid + u * FW_THREADS;
+      bool neg = false;
+      keys[u] = j < n ? fixed_weight_key(seed, stream, (uint32_t)j, neg) : ~0ull;
+      negs |= (uint32_t)neg << u;
+    }
+  }
   uint64_t prefix = 0;  // top bits of the h-th smallest key, 8 more per pass
   int need = h;         // its rank among the keys that share those bits
+  int shift = 0;        // low bits not resolved when selection stopped
   if (h > 0 && h < n) {
     for (int pass = 0; pass < 8; pass++) {
-      const int shift = 56 - 8 * pass;
+      shift = 56 - 8 * pass;
       for (int i = tid; i < 256; i += FW_THREADS) hist[i] = 0;
       __syncthreads();
-      for (int j = tid; j < n; j += FW_THREADS) {
-        bool neg;
-        const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
-        if (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
-          atomicAdd(&hist[(key >> shift) & 255], 1u);
+      if constexpr (KPT > 0) {
+#pragma unroll
+        for (int u = 0; u < KPT; u++) {
+          const uint64_t key = keys[u];
+          if (tid + u * FW_THREADS < n &&
+              (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8))))
+            atomicAdd(&hist[(key >> shift) & 255], 1u);
+        }
+      } else {
+        for (int j = tid; j < n; j += FW_THREADS) {
+          bool neg;
+          const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+          if (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
+            atomicAdd(&hist[(key >> shift) & 255], 1u);
+        }
       }
       __syncthreads();
       if (tid == 0) {
@@ -109,18 +140,28 @@ __global__ void __launch_bounds__(FW_THREADS)
           cum += (int)hist[d];
         }
         s_need = need - cum;
+        s_done = (need - cum) == (int)hist[d];
         s_prefix = prefix | ((uint64_t)d << shift);
       }
       __syncthreads();
       prefix = s_prefix;
       need = s_need;
+      const bool done = s_done != 0;
       __syncthreads();  // hist and the broadcast slots are reused by the next pass
+      if (done) break;  // uniform
     }
   }
-  for (int j = tid; j < n; j += FW_THREADS) {
+  const uint64_t bound = prefix >> shift;
+  for (int u = 0, j = tid; j < n; u++, j += FW_THREADS) {
     bool neg;
-    const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
-    const bool sel = h >= n || (h > 0 && key <= prefix);
+    uint64_t key;
+    if constexpr (KPT > 0) {
+      key = keys[u < KPT ? u : 0];
+      neg = (negs >> u) & 1;
+    } else {
+      key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+    }
+    const bool sel = h >= n || (h > 0 && (key >> shift) <= bound);
     const int e = sel ? (neg ? -1 : 1) : 0;
     const int slot = j < ng ? j : j - ng;
     for (int r = 0; r < nres; r++) {
@@ -169,8 +210,12 @@ extern "C" int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint
       if (2 * ng > 65536) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fixed-weight sampler: n > 65536");
       const int h = kind == CUPOLY_ZO ? ng : (2 * ng < CUPOLY_HAMMINGWEIGHT_H ? 2 * ng
                                                                            : CUPOLY_HAMMINGWEIGHT_H);
-      hipLaunchKernelGGL(k_sample_fixed_weight, dim3(1), dim3(FW_THREADS), 0, c->stream,
-                         (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
+      if (2 * ng <= 16 * FW_THREADS)
+        hipLaunchKernelGGL(k_sample_fixed_weight<16>, dim3(1), dim3(FW_THREADS), 0, c->stream,
+                           (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
+      else
+        hipLaunchKernelGGL(k_sample_fixed_weight<0>, dim3(1), dim3(FW_THREADS), 0, c->stream,
+                           (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
       break;
     }
     default: return cp_fail(CUPOLY_ERR_ARG, "Invalid sampler: " + std::to_string(kind));

@@ -220,7 +220,9 @@ SMALL_KIND = {cp.TERNARY: 2, cp.DISCRETE_GAUSSIAN: 3, cp.NARROW: 4, cp.BINARY: 5
 @pytest.mark.parametrize("kind,n", [(cp.UNIFORM, 4096), (cp.TERNARY, 4096),
                                     (cp.DISCRETE_GAUSSIAN, 4096), (cp.NARROW, 4096),
                                     (cp.BINARY, 4096), (cp.ZO, 4096), (cp.HAMMINGWEIGHT, 4096),
-                                    (cp.ZO, 65536), (cp.HAMMINGWEIGHT, 32), (cp.ZO, 32)])
+                                    (cp.ZO, 65536), (cp.HAMMINGWEIGHT, 32), (cp.ZO, 32),
+                                    (cp.ZO, 16384), (cp.HAMMINGWEIGHT, 16384),
+                                    (cp.HAMMINGWEIGHT, 32768)])
 def test_sampler_matches_oracle_counters(gpu, kind, n):
     k = 3 if n <= 16384 else 1
     e, ctx = engine(k, k + 1, n)
