@@ -70,6 +70,8 @@ SIGNATURES = {
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_reserve": ([_vp, _int], _int),
+    "cupoly_polymul_set_tuning": ([_vp, _int, _int], _int),
+    "cupoly_polymul_autotune": ([_vp, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
     "cupoly_crt": ([_vp, _vp, _vp, _int, _int, _int], _int),
     "cupoly_icrt": ([_vp, _vp, _vp], _int),
@@ -367,6 +369,28 @@ def hom_mul_gather(ctx, outs, ct1s, ct2s):
     p1 = arr(*[_ptr(t).value for t in ct1s])
     p2 = arr(*[_ptr(t).value for t in ct2s])
     _check(lib().cupoly_hom_mul_gather(ctx.handle, po, p1, p2, b))
+
+
+def polymul_set_tuning(ctx, rows=0, persistent=0):
+    """Launch choices of polymul_batch on ctx: rows per workgroup (n = 2^11 ..
+    2^13) and, for n = 2^15, persistent (1) or two-kernel split (-1); 0 keeps
+    the built-in choice."""
+    _check(lib().cupoly_polymul_set_tuning(ctx.handle, int(rows), int(persistent)))
+
+
+def polymul_autotune(ctx, out, a, b, reps=3):
+    """Time every launch choice of polymul_batch for ctx's ring degree on these
+    operands, keep the fastest on ctx and return it as {"rows", "persistent"};
+    out receives the product."""
+    e = ctx.engine
+    batch = a.shape[0]
+    for t in (out, a, b):
+        if tuple(t.shape) != (batch, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: polymul_autotune operand shape %s" % (tuple(t.shape),))
+    r, pe = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().cupoly_polymul_autotune(ctx.handle, _ptr(out), _ptr(a), _ptr(b), batch, int(reps),
+                                         ctypes.byref(r), ctypes.byref(pe)))
+    return {"rows": r.value, "persistent": pe.value}
 
 
 def hom_mul_reserve(ctx, batch):

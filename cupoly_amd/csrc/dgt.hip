@@ -25,6 +25,9 @@
 //
 // Sizes: N = 16..8192 in one kernel; N = 16384, 32768 in two kernels
 // (stages [0,S) on strided columns, then [S, log N) on contiguous blocks).
+#include <utility>
+#include <vector>
+
 #include "dgt_dev.hpp"
 
 // -------------------------------------------------------------------------
@@ -355,6 +358,14 @@ static int setup_size() {
     if ((rc = set_lds(k_dgt_single<LOGN, R, true>, lds))) return rc;
     if ((rc = set_lds(k_dgt_single<LOGN, R, false>, lds))) return rc;
     if ((rc = set_lds(k_polymul_fused<LOGN, R>, lds))) return rc;
+    if constexpr (LOGN >= 10 && LOGN <= 12) {
+      if ((rc = set_lds(k_polymul_fused<LOGN, 1>, lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
+      if ((rc = set_lds(k_polymul_fused<LOGN, 2>, 2 * lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
+    }
+    if constexpr (LOGN >= 10 && LOGN <= 11)
+      if ((rc = set_lds(k_polymul_fused<LOGN, 4>, 4 * lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
+    if constexpr (LOGN == 10)
+      if ((rc = set_lds(k_polymul_fused<LOGN, 8>, 8 * lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
   } else {
     const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     if ((rc = set_lds(k_dgt_cols<LOGN, true>, lds))) return rc;
@@ -440,23 +451,49 @@ int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int
   }
 }
 
+template <int LOGN, int R>
+static void launch_fused(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nunits) {
+  static_assert(R * (1 << LOGN) <= 8192, "a workgroup holds at most 8192 slots");
+  constexpr int T = (1 << LOGN) / EPT;
+  cupoly_engine *e = c->e;
+  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
+  hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
+                     c->stream, out, a, b, nunits, e->k, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
+                     e->d_twist_fwd, e->d_twist_inv);
+}
+
+// Rows (residue polynomials) per workgroup the batched product can use at
+// this size besides the built-in rows_for: n = 2^11 .. 2^13 only.
+static bool fused_rows_ok(int logng, int rows) {
+  if (logng < 10 || logng > 12) return false;
+  return (rows == 1 || rows == 2 || rows == 4 || rows == 8) && (rows << logng) <= 8192;
+}
+
 template <int LOGN>
 static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch) {
   cupoly_engine *e = c->e;
   const int nres = e->k, nunits = nres * batch;
   if (nunits <= 0) return CUPOLY_OK;
+  const bool persistent =
+      c->tune_persistent ? c->tune_persistent > 0 : nunits >= e->ncu;
   if constexpr (LOGN <= 13) {
-    constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
-    const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
-    hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
-                       c->stream, out, a, b, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
-                       e->d_twist_fwd, e->d_twist_inv);
+    const int rows = fused_rows_ok(LOGN, c->tune_rows) ? c->tune_rows : rows_for<LOGN>();
+    if constexpr (LOGN >= 10 && LOGN <= 12) {
+      if (rows == 1) launch_fused<LOGN, 1>(c, out, a, b, nunits);
+      else if (rows == 2) launch_fused<LOGN, 2>(c, out, a, b, nunits);
+      else if constexpr (LOGN <= 11) {
+        if (rows == 4) launch_fused<LOGN, 4>(c, out, a, b, nunits);
+        else if constexpr (LOGN == 10) launch_fused<LOGN, 8>(c, out, a, b, nunits);
+      }
+    } else {
+      launch_fused<LOGN, rows_for<LOGN>()>(c, out, a, b, nunits);
+    }
     CP_LAUNCH_CHECK();
-  } else if (LOGN == 14 && nunits >= e->ncu) {
+  } else if (LOGN == 14 && persistent) {
     // persistent: one workgroup per CU, each with its own 384 KiB park area.
-    // Fewer units than CUs keep the split below, which spreads each product
-    // over twice as many workgroups.
-    const int grid = e->ncu;
+    // By default batches with fewer units than CUs keep the split below,
+    // which spreads each product over twice as many workgroups.
+    const int grid = nunits < e->ncu ? nunits : e->ncu;
     void *ws;
     int rc = cp_workspace(c, (size_t)grid * 3 * (1 << (LOGN - 1)) * sizeof(cp_gi), &ws);
     if (rc) return rc;
@@ -525,4 +562,65 @@ extern "C" int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
                                     const cupoly_gi *b, int batch) {
   if (!c || !out || !a || !b || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "polymul_batch: bad argument");
   return cp_polymul_fused_launch(c, (cp_gi *)out, (const cp_gi *)a, (const cp_gi *)b, batch);
+}
+
+extern "C" int cupoly_polymul_set_tuning(cupoly_ctx *c, int rows, int persistent) {
+  if (!c) return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: null context");
+  if (rows != 0 && !fused_rows_ok(c->e->logng, rows))
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: rows " + std::to_string(rows) +
+                                       " not available at this ring degree");
+  if (persistent < -1 || persistent > 1)
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: persistent must be -1, 0 or 1");
+  if (persistent != 0 && c->e->logng != 14)
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: the persistent choice exists for n = 2^15 only");
+  c->tune_rows = rows;
+  c->tune_persistent = persistent;
+  return CUPOLY_OK;
+}
+
+extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
+                                       const cupoly_gi *b, int batch, int reps, int *rows,
+                                       int *persistent) {
+  if (!c || !out || !a || !b || batch < 1 || reps < 1)
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_autotune: bad argument");
+  // candidate (rows, persistent) pairs for this ring degree
+  std::vector<std::pair<int, int>> cand;
+  const int logng = c->e->logng;
+  for (int r : {1, 2, 4, 8})
+    if (fused_rows_ok(logng, r)) cand.push_back({r, 0});
+  if (logng == 14) cand = {{0, 1}, {0, -1}};
+  if (cand.empty()) cand.push_back({0, 0});
+  hipEvent_t e0, e1;
+  CP_HIP(hipEventCreate(&e0));
+  CP_HIP(hipEventCreate(&e1));
+  auto hip = [](hipError_t err, const char *what) {
+    return err == hipSuccess ? CUPOLY_OK : cp_check_hip(err, what);
+  };
+  float best = -1.0f;
+  std::pair<int, int> pick = cand[0];
+  int rc = CUPOLY_OK;
+  for (const auto &cd : cand) {
+    c->tune_rows = cd.first;
+    c->tune_persistent = cd.second;
+    if ((rc = cupoly_polymul_batch(c, out, a, b, batch))) break;  // warm-up
+    if ((rc = hip(hipEventRecord(e0, c->stream), "hipEventRecord"))) break;
+    for (int i = 0; i < reps && !rc; i++) rc = cupoly_polymul_batch(c, out, a, b, batch);
+    if (rc || (rc = hip(hipEventRecord(e1, c->stream), "hipEventRecord")) ||
+        (rc = hip(hipEventSynchronize(e1), "hipEventSynchronize")))
+      break;
+    float ms = 0.0f;
+    if ((rc = hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))) break;
+    if (best < 0.0f || ms < best) {
+      best = ms;
+      pick = cd;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->tune_rows = pick.first;
+  c->tune_persistent = pick.second;
+  if (rc) return rc;
+  if (rows) *rows = pick.first;
+  if (persistent) *persistent = pick.second;
+  return cupoly_polymul_batch(c, out, a, b, batch);
 }

@@ -80,6 +80,10 @@ struct cupoly_ctx {
   hipEvent_t ring_ev[kRing] = {};
   bool ring_used[kRing] = {};
   int ring_next = 0;
+  // batched-product launch choices (cupoly_polymul_set_tuning / _autotune);
+  // 0 = the built-in choice
+  int tune_rows = 0;        // residue polynomials per workgroup, n = 2^11 .. 2^13
+  int tune_persistent = 0;  // n = 2^15: 1 one workgroup per product, -1 two-kernel split
 };
 
 // error plumbing

@@ -181,6 +181,19 @@ int cupoly_dot(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *const *a, const c
  * launch: forward DGT of a and b, pointwise product, inverse DGT. */
 int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
                          int batch);
+/* Launch choices of cupoly_polymul_batch on this context (no reference
+ * counterpart): `rows` residue polynomials per workgroup for n = 2^11 .. 2^13
+ * (1, 2, 4 or 8 with rows * n/2 <= 8192; 0 = built-in), and for n = 2^15
+ * `persistent` = 1 (one workgroup per residue product, persistent) or -1
+ * (two-kernel column/block split; 0 = persistent when the batch has at least
+ * one residue product per CU). */
+int cupoly_polymul_set_tuning(cupoly_ctx *c, int rows, int persistent);
+/* Autotuner for the batched product: times every available (rows,
+ * persistent) choice for this ring degree over `reps` launches on the
+ * caller's operands, keeps the fastest on the context, reports it in *rows /
+ * *persistent (either may be NULL) and leaves the product in out. */
+int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
+                            int batch, int reps, int *rows, int *persistent);
 /* The same products one launch per op, in the reference's order
  * (fwd a, fwd b, mul, inv) through cupoly_dgt / cupoly_mul_dgt_gi; a and b are
  * left in the DGT domain afterwards. */

@@ -295,3 +295,41 @@ def test_sample_muladd_matches_oracle(gpu, k, n, batch):
     out = cp.empty(a.shape)
     cp.sample_muladd_batch(ctx, out, cp.to_device(a), 4242, 10)
     assert np.array_equal(cp.to_host(out), O.sample_muladd(P, a, 4242, 10))
+
+
+@pytest.mark.parametrize("n,k,batch,choices", [
+    (2048, 2, 40, [(1, 0), (2, 0), (4, 0), (8, 0)]),
+    (4096, 3, 20, [(1, 0), (2, 0), (4, 0)]),
+    (8192, 2, 12, [(1, 0), (2, 0)]),
+    (32768, 3, 90, [(0, 1), (0, -1)]),
+])
+def test_polymul_tuning_choices_and_autotune(gpu, n, k, batch, choices):
+    """Every launch choice of the batched product (rows per workgroup; for
+    n = 2^15 persistent or split) gives the same products, equal to the
+    oracle; the autotuner picks one of them and leaves the product in out."""
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 500 + n)
+    b = rand_res(P, P.q, (batch,), 501 + n)
+    da, db = cp.to_device(a), cp.to_device(b)
+    want = None
+    for rows, persistent in choices:
+        cp.polymul_set_tuning(ctx, rows, persistent)
+        out = cp.empty(a.shape)
+        cp.polymul_batch(ctx, out, da, db)
+        h = cp.to_host(out)
+        if want is None:
+            for i in (0, batch - 1):
+                assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
+            want = h
+        assert np.array_equal(h, want), (rows, persistent)
+    out = cp.empty(a.shape)
+    pick = cp.polymul_autotune(ctx, out, da, db, reps=2)
+    assert (pick["rows"], pick["persistent"]) in choices
+    assert np.array_equal(cp.to_host(out), want)
+    cp.polymul_set_tuning(ctx, 0, 0)
+    with pytest.raises(RuntimeError):
+        cp.polymul_set_tuning(ctx, 16, 0)
+    if n != 32768:
+        with pytest.raises(RuntimeError):
+            cp.polymul_set_tuning(ctx, 0, 1)
