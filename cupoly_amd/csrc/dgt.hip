@@ -203,10 +203,9 @@ __global__ void __launch_bounds__(1024)
 // Fused coefficient-domain product c = a*b, N <= 8192: forward(a) and
 // forward(b) in one workgroup, slot-wise Gaussian product, inverse.
 // For N = 4096 (512 threads, 72 KiB of LDS) two workgroups fit on a CU only
-// if each wave keeps to 128 VGPRs, so that size asks for 4 waves per SIMD.
+// if each wave keeps to 128 VGPRs, so that size asks for 4 waves per SIMD
+// (fused_min_waves).
 // -------------------------------------------------------------------------
-template <int LOGN>
-constexpr int fused_min_waves() { return LOGN == 12 ? 4 : 1; }
 
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())

@@ -451,6 +451,12 @@ __device__ __forceinline__ void twist(cp_gi &v, const CP_GLOBAL uint64_t *__rest
 }
 
 
+// Minimum waves per SIMD asked of a one-transform-per-workgroup kernel: at
+// N = 4096 slots (512 threads, 72 KiB of LDS) two workgroups share a CU only
+// if each wave keeps to 128 VGPRs.
+template <int LOGN>
+constexpr int fused_min_waves() { return LOGN == 12 ? 4 : 1; }
+
 // Residue polynomials per workgroup for a single-kernel transform of 2^LOGN
 // slots: at least 256 threads per workgroup.
 template <int LOGN>

@@ -4,10 +4,16 @@
 // into poly_t objects (Sampler::sample, src/cuda/sampler.cu:576-635, each
 // followed by a forward DGT) and then calls poly_mul_add (polynomial.cu:916-948
 // -> muladd_dgt, dgt.cu:1439-1499): five launches and four polynomials through
-// HBM.  Here one workgroup per (instance, residue) draws s (ternary) and e
-// (discrete Gaussian) from their Philox counters, twists and transforms both
-// in LDS, multiplies by a (read once, in the DGT domain) and writes out: the
-// HBM traffic is a in and out out, 16 k n bytes per instance.
+// HBM.  Here two launches:
+//   1. k_sample_se draws s (ternary) and e (discrete Gaussian) of every
+//      instance once from their Philox counters (they are the same integers
+//      in every residue) and packs each Gaussian slot's four small values
+//      (s re, s imag, e re, e imag) as signed bytes: 4 bytes per slot;
+//   2. one workgroup per (instance, residue) reads its slots' packed values,
+//      reduces them mod q_r, twists and transforms s and e in LDS, multiplies
+//      by a (read once, in the DGT domain) and writes out.
+// HBM traffic: a in and out out, 16 k n bytes per instance, plus the packed
+// samples (2 n bytes written once, read k times).
 //
 // Instance i uses Philox stream stream0 + 2i for s and stream0 + 2i + 1 for e,
 // so the small polynomials are the ones cupoly_sample_coeff draws for those
@@ -15,12 +21,30 @@
 #include "dgt_dev.hpp"
 #include "philox.hpp"
 
+// one thread per (instance, Gaussian slot g): s and e coefficients g and g + ng
+__global__ void __launch_bounds__(256)
+    k_sample_se(uint32_t *__restrict__ se, int ng, long total, uint64_t seed, uint64_t stream0,
+                cp_cdt cdt) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const uint64_t inst = (uint64_t)(idx / ng);
+    const uint32_t g = (uint32_t)(idx % ng);
+    const int s0 = ternary_coef(seed, stream0 + 2 * inst, g);
+    const int s1 = ternary_coef(seed, stream0 + 2 * inst, g + (uint32_t)ng);
+    const int e0 = gaussian_coef(seed, stream0 + 2 * inst + 1, g, cdt);
+    const int e1 = gaussian_coef(seed, stream0 + 2 * inst + 1, g + (uint32_t)ng, cdt);
+    se[idx] = (uint32_t)(uint8_t)s0 | ((uint32_t)(uint8_t)s1 << 8) |
+              ((uint32_t)(uint8_t)e0 << 16) | ((uint32_t)(uint8_t)e1 << 24);
+  }
+}
+
+__device__ __forceinline__ int se_byte(uint32_t w, int k) { return (int)(int8_t)(w >> (8 * k)); }
+
 template <int LOGN, int ROWS>
-__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_sample_muladd(cp_gi *__restrict__ out, const cp_gi *__restrict__ a, int nunits, int nres,
                     const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                    const uint64_t *__restrict__ tsf_all, uint64_t seed, uint64_t stream0,
-                    cp_cdt cdt) {
+                    const uint64_t *__restrict__ tsf_all, const uint32_t *__restrict__ se_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   using LP = LastPass<LOGN, 0, LOGN>;
   extern __shared__ cp_gi lds_all[];
@@ -38,14 +62,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   const uint64_t *twf = twf_all + (size_t)rid * N;
   const uint64_t *tsf = tsf_all + (size_t)rid * N * 6;
   const size_t off = (size_t)unit * N;
+  const uint32_t *se = se_all + inst * N;
   MapIdentity id;
   cp_gi x[EPT], y[EPT];
   // s: ternary, coefficient g in re and g + N in imag of slot g
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<LOGN, 0, LOGN>(t, i);
-    x[i].re = small_mod(ternary_coef(seed, stream0 + 2 * inst, (uint32_t)g), p);
-    x[i].imag = small_mod(ternary_coef(seed, stream0 + 2 * inst, (uint32_t)(g + N)), p);
+    const uint32_t w = se[g];
+    x[i].re = small_mod(se_byte(w, 0), p);
+    x[i].imag = small_mod(se_byte(w, 1), p);
     twist(x[i], tsf, g, p);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -60,8 +86,9 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<LOGN, 0, LOGN>(te, i);
-    y[i].re = small_mod(gaussian_coef(seed, stream0 + 2 * inst + 1, (uint32_t)g, cdt), p);
-    y[i].imag = small_mod(gaussian_coef(seed, stream0 + 2 * inst + 1, (uint32_t)(g + N), cdt), p);
+    const uint32_t w = se[g];
+    y[i].re = small_mod(se_byte(w, 2), p);
+    y[i].imag = small_mod(se_byte(w, 3), p);
     twist(y[i], tsf, g, p);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -90,9 +117,19 @@ static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
+  // packed samples of every instance: 4 bytes per Gaussian slot
+  const long total = (long)batch * e->ng;
+  void *ws;
+  int rc = cp_workspace(c, (size_t)total * sizeof(uint32_t), &ws);
+  if (rc) return rc;
+  long g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_sample_se, dim3((unsigned)g), dim3(256), 0, c->stream, (uint32_t *)ws,
+                     e->ng, total, seed, stream0, cp_make_cdt());
+  CP_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_sample_muladd<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
                      c->stream, out, a, nunits, nres, e->d_primes, e->d_tw_fwd, e->d_twist_fwd,
-                     seed, stream0, cp_make_cdt());
+                     (const uint32_t *)ws);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }

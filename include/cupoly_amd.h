@@ -225,7 +225,9 @@ int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
  * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian
  * (stream stream0 + 2i + 1), both drawn on the device (Sampler::sample x2 +
  * poly_mul_add, sampler.cu:576-635, polynomial.cu:916-948); a, out
- * [batch][k][n/2].  One fused launch for n <= 16384. */
+ * [batch][k][n/2].  For n <= 16384 two launches: s_i and e_i drawn once per
+ * instance into the context's workspace (4 bytes per slot), then one fused
+ * transform + multiply-add launch per residue. */
 int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, int batch,
                                uint64_t seed, uint64_t stream0);
 
