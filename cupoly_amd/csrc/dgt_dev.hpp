@@ -535,7 +535,8 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
   }
 }
 
-// `park` must hold 3 * 8192 slots for this workgroup.
+// `park` must hold 3 * 8192 slots for this workgroup.  The halves run in
+// loops that are not unrolled, so each pass sequence is compiled once.
 __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
                                                  const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
                                                  cp_gi *lds, const cp_prime &pr,
@@ -550,29 +551,30 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   // a: spectrum halves to P0, P1
   stage0_16k(x, y, a, t, twf, tsf, p);
   park_st(park, 1, t, y);
-  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
-  park_st(park, 0, t, x);
-  park_ld(park, 1, t, y);
-  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H});
-  park_st(park, 1, t, y);
-  // b, half 0: spectrum, product with P0, inverse stages 13..1, back to P0
+#pragma nounroll
+  for (int h = 0; h < 2; h++) {
+    if (h == 1) park_ld(park, 1, t, x);
+    dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{h * H});
+    park_st(park, h, t, x);
+  }
+  // b: per half, spectrum, product with a's (P0 / P1), inverse stages 13..1;
+  // half 0's result waits in P0
   int tb = t;
   asm volatile("" : "+v"(tb));
   stage0_16k(x, y, b, tb, twf, tsf, p);
   park_st(park, 2, t, y);
-  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
-  park_ld(park, 0, t, y);
+#pragma nounroll
+  for (int h = 0; h < 2; h++) {
+    if (h == 1) park_ld(park, 2, t, x);
+    dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{h * H});
+    park_ld(park, h, t, y);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = cp_gimul(x[i], y[i], pr);
-  dit_passes<14, 13, 0, 13, 1>(x, lds, t, twi, p, MapBlock{0});
-  park_st(park, 0, t, x);
-  // b, half 1 with P1
-  park_ld(park, 2, t, y);
-  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H});
-  park_ld(park, 1, t, x);
+    for (int i = 0; i < EPT; i++) x[i] = cp_gimul(x[i], y[i], pr);
+    dit_passes<14, 13, 0, 13, 1>(x, lds, t, twi, p, MapBlock{h * H});
+    if (h == 0) park_st(park, 0, t, x);
+  }
 #pragma unroll
-  for (int i = 0; i < EPT; i++) y[i] = cp_gimul(y[i], x[i], pr);
-  dit_passes<14, 13, 0, 13, 1>(y, lds, t, twi, p, MapBlock{H});
+  for (int i = 0; i < EPT; i++) y[i] = x[i];
   park_ld(park, 0, t, x);
   int to = t;
   asm volatile("" : "+v"(to));
