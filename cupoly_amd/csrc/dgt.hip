@@ -559,6 +559,8 @@ extern "C" int cupoly_dgt(cupoly_ctx *c, cupoly_gi *data, int base, int directio
 
 extern "C" int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
                                     const cupoly_gi *b, int batch) {
+  // an empty batch is a no-op (its operand pointers may be null)
+  if (c && batch == 0) return CUPOLY_OK;
   if (!c || !out || !a || !b || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "polymul_batch: bad argument");
   return cp_polymul_fused_launch(c, (cp_gi *)out, (const cp_gi *)a, (const cp_gi *)b, batch);
 }

@@ -136,8 +136,8 @@ static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch,
 
 extern "C" int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
                                           int batch, uint64_t seed, uint64_t stream0) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: pointers may be null
   if (!c || !out || !a || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "sample_muladd: bad argument");
-  if (batch == 0) return CUPOLY_OK;
   switch (c->e->logng) {
     case 4: return sample_muladd_t<4>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
     case 5: return sample_muladd_t<5>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);

@@ -327,9 +327,9 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
 
 extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
                                     const cupoly_gi *ct2, int batch) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: pointers may be null
   if (!c || !out || !ct1 || !ct2 || batch < 0)
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch: bad argument");
-  if (batch == 0) return CUPOLY_OK;
   const size_t k = c->e->k, ng = c->e->ng;
   cp_cts o{(cp_gi *)out, 3 * k * ng, nullptr}, a{(cp_gi *)ct1, 2 * k * ng, nullptr},
       b{(cp_gi *)ct2, 2 * k * ng, nullptr};
@@ -339,9 +339,9 @@ extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
 extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
                                      const cupoly_gi *const *ct1, const cupoly_gi *const *ct2,
                                      int batch) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: tables may be null
   if (!c || !out || !ct1 || !ct2 || batch < 0)
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
-  if (batch == 0) return CUPOLY_OK;
   for (int i = 0; i < batch; i++)
     if (!out[i] || !ct1[i] || !ct2[i]) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: null entry");
   // pointer tables to the device on the context's stream, from a pinned

@@ -201,6 +201,7 @@ extern "C" int cupoly_double_op_by_int(cupoly_ctx *c, cupoly_gi *b1, const cupol
 // inverse in poly_copy_to_host): forward(a), forward(b), mul, inverse(c).
 extern "C" int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi *b,
                                     int batch) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: pointers may be null
   if (!c || !out || !a || !b || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "polymul_plain: bad argument");
   const size_t stride = (size_t)c->e->k * c->e->ng;
   for (int i = 0; i < batch; i++) {
