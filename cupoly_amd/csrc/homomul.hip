@@ -141,18 +141,57 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Tensor product fused into the inverse transform (n <= 2^14): one
-// workgroup row per output residue polynomial (pair, m, r) reads the DGT-
-// domain operands X[pair][0..3][r] in natural order, forms
-// m = 0: c0 d0,  m = 1: c0 d1 + c1 d0,  m = 2: c1 d1
-// slot-wise on its way into LDS, then runs the inverse DGT and untwist.
+// Forward DGT of the pipeline's extended operands (n <= 2^14), spectrum kept
+// in thread-major order: slot i of thread t of a residue polynomial goes to
+// X[unit][i * T + t], where the inverse below reads it back into the same
+// register of the same thread (its first inverse pass starts in the forward's
+// last-pass layout).  X is private to the pipeline, so neither side needs the
+// LDS exchange that puts a spectrum into position order, and every global
+// access is lane-contiguous.
 template <int LOGN, int ROWS>
-__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
+    k_dgt_fwd_tm(cp_gi *__restrict__ X, int nunits, int nres, const cp_prime *__restrict__ primes,
+                 const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ tsf_all) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  extern __shared__ cp_gi lds_all[];
+  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
+  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
+  cp_gi *lds = lds_all + row * lds_row<LOGN>();
+  int unit = blockIdx.x * ROWS + row;
+  const bool active = unit < nunits;
+  if (!active) unit = nunits - 1;
+  int rid = unit % nres;
+  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
+  const uint64_t p = primes[rid].p;
+  const uint64_t *tw = twf_all + (size_t)rid * N, *tt = tsf_all + (size_t)rid * N * 6;
+  cp_gi *poly = X + (size_t)unit * N;
+  cp_gi x[EPT];
+  MapIdentity id;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<LOGN, 0, LOGN>(t, i);
+    x[i] = poly[g];
+    twist(x[i], tt, g, p);
+  }
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
+  // every thread has loaded its inputs before the first exchange barrier
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) poly[i * T + t] = x[i];
+  }
+}
+
+// Tensor product fused into the inverse transform (n <= 2^14): one
+// workgroup row per output residue polynomial (pair, m, r) reads the
+// thread-major spectra X[pair][0..3][r] written by k_dgt_fwd_tm, forms
+// m = 0: c0 d0,  m = 1: c0 d1 + c1 d0,  m = 2: c1 d1
+// slot-wise in registers, then runs the inverse DGT and untwist.
+template <int LOGN, int ROWS>
+__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_tensor_idgt(cp_gi *__restrict__ Y, const cp_gi *__restrict__ X, int nunits, int nres,
                   const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twi_all,
                   const uint64_t *__restrict__ tsi_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
-  using LP = LastPass<LOGN, 0, LOGN>;
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
@@ -168,20 +207,17 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
   const size_t per_pair = (size_t)nres * N;
   const cp_gi *x = X + (size_t)pair * 4 * per_pair + (size_t)rid * N;
   const cp_gi *xa = x + (m == 2 ? per_pair : 0), *xb = x + (m == 0 ? 2 : 3) * per_pair;
+  cp_gi v[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
-    const int g = nat_index<LOGN>(t, i);
-    cp_gi v = cp_gimul(xa[g], xb[g], pr);
+    const int g = i * T + t;  // thread-major
+    v[i] = cp_gimul(xa[g], xb[g], pr);
     if (m == 1) {  // + c1 d0
       const cp_gi w = cp_gimul(x[per_pair + g], x[2 * per_pair + g], pr);
-      v.re = cp_add(v.re, w.re, p);
-      v.imag = cp_add(v.imag, w.imag, p);
+      v[i].re = cp_add(v[i].re, w.re, p);
+      v[i].imag = cp_add(v[i].imag, w.imag, p);
     }
-    lds[lds_phys(g, LP::MS)] = v;
   }
-  __syncthreads();
-  cp_gi v[EPT];
-  lds_load<LOGN, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, v, t);
   const uint64_t *twi = twi_all + (size_t)rid * N, *tsi = tsi_all + (size_t)rid * N * 6;
   MapIdentity id;
   dit_passes<LOGN, LOGN, 0, LOGN, 0>(v, lds, t, twi, p, id);
@@ -193,6 +229,33 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
       twist(v[i], tsi, g, p);
       y[g] = v[i];
     }
+  }
+}
+
+template <int LOGN>
+static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits) {
+  constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
+  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
+  static bool attr = false;
+  if (!attr) {
+    CP_HIP(hipFuncSetAttribute((const void *)k_dgt_fwd_tm<LOGN, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  cupoly_engine *e = c->e;
+  hipLaunchKernelGGL((k_dgt_fwd_tm<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
+                     c->stream, X, nunits, e->nres, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
+static int fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits) {
+  switch (c->e->logng) {
+    case 10: return launch_fwd_tm<10>(c, X, nunits);
+    case 11: return launch_fwd_tm<11>(c, X, nunits);
+    case 12: return launch_fwd_tm<12>(c, X, nunits);
+    case 13: return launch_fwd_tm<13>(c, X, nunits);
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fwd_tm: ring degree");
   }
 }
 
@@ -310,10 +373,11 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1, ct2,
                      ng, e->d_hps);
   CP_LAUNCH_CHECK();
-  if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
   if (e->logng >= 10 && e->logng <= 13) {
+    if ((rc = fwd_tm(c, X, batch * 4 * nres))) return rc;
     if ((rc = tensor_idgt(c, Y, X, batch * 3 * nres))) return rc;
   } else {
+    if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
     const long total = (long)batch * nres * ng;
     long g = (total + 255) / 256;
     if (g > 4096) g = 4096;
