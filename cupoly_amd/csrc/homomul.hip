@@ -10,8 +10,10 @@
 // poly_complex_scaling_tDivQ (polynomial.cu:350-404, cudaengine.cu:382-796).
 //
 // Pipeline, every stage batched over all ciphertext pairs (one launch each):
-//   1. k_ext_q_to_qb:   each of the 4 inputs -> QB residues (Q copied, B by the
-//                       HPS extension), one thread per Gaussian slot;
+//   1. k_ext_q_to_qb:   each of the 4 inputs -> its B residues by the HPS
+//                       extension, one thread per Gaussian slot (the Q rows
+//                       are read from the input by the forward transform,
+//                       or copied for n = 2^15);
 //   2. forward DGT of the 4*(k+kl) residue polynomials per pair;
 //   3.+4. k_tensor_idgt: slot-wise c0 d0, c0 d1 + c1 d0, c1 d1 formed as the
 //                       inverse DGT of each of the 3*(k+kl) results loads its
@@ -35,14 +37,15 @@ struct cp_cts {
 // blockIdx.y = pair * 4 + which (0: c0, 1: c1, 2: d0, 3: d1)
 __global__ void __launch_bounds__(256)
     k_ext_q_to_qb(cp_gi *__restrict__ X, cp_cts ct1, cp_cts ct2, int ng,
-                  const cp_hps *__restrict__ H) {
+                  const cp_hps *__restrict__ H, bool copy_q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int k = H->k, kl = H->kl;
   const int pair = blockIdx.y >> 2, which = blockIdx.y & 3;
   const cp_gi *in = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)(which & 1) * k * ng;
   cp_gi *out = X + (size_t)blockIdx.y * (k + kl) * ng;
-  for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
+  if (copy_q)  // else the forward transform reads the Q rows from the input itself
+    for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
   hps_q_to_b_slot(in, ng, out + (size_t)k * ng, ng, c, H);
 }
 
@@ -150,8 +153,9 @@ __global__ void __launch_bounds__(256)
 // access is lane-contiguous.
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
-    k_dgt_fwd_tm(cp_gi *__restrict__ X, int nunits, int nres, const cp_prime *__restrict__ primes,
-                 const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ tsf_all) {
+    k_dgt_fwd_tm(cp_gi *__restrict__ X, int nunits, int nres, int k, cp_cts ct1, cp_cts ct2,
+                 const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                 const uint64_t *__restrict__ tsf_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
@@ -165,12 +169,19 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const uint64_t p = primes[rid].p;
   const uint64_t *tw = twf_all + (size_t)rid * N, *tt = tsf_all + (size_t)rid * N * 6;
   cp_gi *poly = X + (size_t)unit * N;
+  // a Q residue comes straight from its ciphertext, a B residue from the
+  // extension written by k_ext_q_to_qb (unit = (pair * 4 + which) * nres + r)
+  const cp_gi *src = poly;
+  if (rid < k) {
+    const int pw = unit / nres, pair = pw >> 2, which = pw & 3;
+    src = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)((which & 1) * k + rid) * N;
+  }
   cp_gi x[EPT];
   MapIdentity id;
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<LOGN, 0, LOGN>(t, i);
-    x[i] = poly[g];
+    x[i] = src[g];
     twist(x[i], tt, g, p);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
@@ -233,7 +244,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 }
 
 template <int LOGN>
-static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits) {
+static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   static bool attr = false;
@@ -244,17 +255,18 @@ static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits) {
   }
   cupoly_engine *e = c->e;
   hipLaunchKernelGGL((k_dgt_fwd_tm<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
-                     c->stream, X, nunits, e->nres, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+                     c->stream, X, nunits, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
+                     e->d_twist_fwd);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }
 
-static int fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits) {
+static int fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
   switch (c->e->logng) {
-    case 10: return launch_fwd_tm<10>(c, X, nunits);
-    case 11: return launch_fwd_tm<11>(c, X, nunits);
-    case 12: return launch_fwd_tm<12>(c, X, nunits);
-    case 13: return launch_fwd_tm<13>(c, X, nunits);
+    case 10: return launch_fwd_tm<10>(c, X, nunits, ct1, ct2);
+    case 11: return launch_fwd_tm<11>(c, X, nunits, ct1, ct2);
+    case 12: return launch_fwd_tm<12>(c, X, nunits, ct1, ct2);
+    case 13: return launch_fwd_tm<13>(c, X, nunits, ct1, ct2);
     default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fwd_tm: ring degree");
   }
 }
@@ -370,11 +382,12 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   if (rc) return rc;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
+  const bool fused_sizes = e->logng >= 10 && e->logng <= 13;
   hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1, ct2,
-                     ng, e->d_hps);
+                     ng, e->d_hps, !fused_sizes);
   CP_LAUNCH_CHECK();
-  if (e->logng >= 10 && e->logng <= 13) {
-    if ((rc = fwd_tm(c, X, batch * 4 * nres))) return rc;
+  if (fused_sizes) {
+    if ((rc = fwd_tm(c, X, batch * 4 * nres, ct1, ct2))) return rc;
     if ((rc = tensor_idgt(c, Y, X, batch * 3 * nres))) return rc;
   } else {
     if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
