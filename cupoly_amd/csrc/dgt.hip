@@ -506,10 +506,11 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     using SP = Split<LOGN>;
     const size_t bytes = (size_t)nunits * (1 << LOGN) * sizeof(cp_gi);
     const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
-    if (out != a) CP_HIP(hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream));
+    // b first: out may be b itself
     cp_gi *tb = nullptr;
     CP_HIP(hipMallocAsync((void **)&tb, bytes, c->stream));
     CP_HIP(hipMemcpyAsync(tb, b, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (out != a) CP_HIP(hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream));
     const int gcols = nunits * SP::WG_PER_POLY;
     const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
     hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, out,

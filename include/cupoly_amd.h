@@ -177,8 +177,9 @@ int cupoly_dot(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *const *a, const c
 /* ---- batched products (no reference counterpart: the reference issues one
  * poly_mul per product, 7 launches each, polynomial.cu:892-914) ------------
  * out[i] = a[i] * b[i] mod (X^n + 1, q_r) for `batch` products of the engine's
- * base-Q shape, coefficient domain in and out, [batch][k][n/2] GIs.  One fused
- * launch: forward DGT of a and b, pointwise product, inverse DGT. */
+ * base-Q shape, coefficient domain in and out, [batch][k][n/2] GIs; out may be
+ * a or b.  One fused launch: forward DGT of a and b, pointwise product,
+ * inverse DGT. */
 int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
                          int batch);
 /* Launch choices of cupoly_polymul_batch on this context (no reference

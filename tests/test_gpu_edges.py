@@ -121,3 +121,25 @@ def test_empty_batches_are_no_ops(gpu):
     o = cp.empty((1, k, ng, 2))
     cp.polymul_batch(ctx, o, cp.to_device(O.fold(x, ng)), cp.to_device(O.fold(one, ng)))
     assert torch.equal(o, cp.to_device(O.fold(x, ng)))
+
+
+@pytest.mark.parametrize("n,k,batch", [(2048, 2, 4), (16384, 4, 4), (32768, 3, 4),
+                                         (32768, 3, 90)])
+def test_product_in_place(gpu, n, k, batch):
+    """out may be a or b itself (each residue product reads its operands
+    before it writes its result; n = 2^15 covers the split and the
+    one-workgroup-per-product forms)."""
+    e, ctx = engine(k, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = O.fold(rand_coefs(P.q, (batch,), n, 31), P.Ng)
+    b = O.fold(rand_coefs(P.q, (batch,), n, 32), P.Ng)
+    want = cp.empty(a.shape)
+    cp.polymul_batch(ctx, want, cp.to_device(a), cp.to_device(b))
+    da, db = cp.to_device(a), cp.to_device(b)
+    cp.polymul_batch(ctx, da, da, db)  # out is a
+    assert torch.equal(da, want)
+    da, db = cp.to_device(a), cp.to_device(b)
+    cp.polymul_batch(ctx, db, da, db)  # out is b
+    assert torch.equal(db, want)
+    i = batch - 1
+    assert np.array_equal(cp.to_host(want)[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
