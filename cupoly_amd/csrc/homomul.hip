@@ -192,6 +192,126 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
 }
 
+// ---- small batches: one thread per (slot, output residue) ------------------
+// A batch of one or two pairs gives the per-slot kernels above only a few
+// waves per SIMD, each running a long serial chain over all residues.  These
+// forms give each output residue its own thread; the per-slot values every
+// residue needs (a_j = x_j (Q/q_j)^-1, the FP64 sums) are recomputed per
+// thread by the same operations in the same order, so every residue is the
+// same as in the per-slot kernels.
+
+// Q -> B, output residue i = blockIdx.z (blockIdx.y = pair * 4 + which)
+__global__ void __launch_bounds__(256)
+    k_ext_q_to_b_wide(cp_gi *__restrict__ X, cp_cts ct1, cp_cts ct2, int ng,
+                      const cp_hps *__restrict__ H, bool copy_q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ng) return;
+  const int k = H->k, kl = H->kl, i = blockIdx.z;
+  const int pair = blockIdx.y >> 2, which = blockIdx.y & 3;
+  const cp_gi *in = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)(which & 1) * k * ng;
+  cp_gi *out = X + (size_t)blockIdx.y * (k + kl) * ng;
+  if (copy_q && i == 0)
+    for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
+  uint64_t ar[CP_MAX_BASE], ai[CP_MAX_BASE];
+  double vr = 0.0, vi = 0.0;
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      const uint64_t q = H->q[j].p;
+      const cp_gi x = in[(size_t)j * ng + c];
+      ar[j] = shoup2(x.re, H->invQqi[j], q);
+      ai[j] = shoup2(x.imag, H->invQqi[j], q);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      const double r = H->rcp_q[j];
+      vr = __ocml_fma_rtn_f64((double)ar[j], r, vr);
+      vi = __ocml_fma_rtn_f64((double)ai[j], r, vi);
+    }
+  }
+  const uint64_t v_re = snap(vr), v_im = snap(vi);
+  const uint64_t b = H->b[i].p;
+  uint64_t sr = 0, si = 0;
+#pragma unroll
+  for (int j = 0; j < CP_MAX_BASE; j++) {
+    if (j < k) {
+      sr = addmod_ref(sr, shoup2(ar[j], H->Qqibi[i][j], b), b);
+      si = addmod_ref(si, shoup2(ai[j], H->Qqibi[i][j], b), b);
+    }
+  }
+  sr = submod_ref(sr, shoup2(v_re, H->Qbi[i], b), b);
+  si = submod_ref(si, shoup2(v_im, H->Qbi[i], b), b);
+  out[(size_t)(k + i) * ng + c] = cp_gi{sr, si};
+}
+
+// t/Q scaling, B residue i = blockIdx.z of output polynomial blockIdx.y, and
+// the first step of B -> Q: Z[pm][i][c] = y_i (B/b_i)^-1 mod b_i.
+template <int K>
+__global__ void __launch_bounds__(256)
+    k_scale_b_wide(cp_gi *__restrict__ Z, const cp_gi *__restrict__ Y, int ng,
+                   const cp_hps *__restrict__ H) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ng) return;
+  const int kl = H->kl, i = blockIdx.z, pm = blockIdx.y;
+  const cp_gi *y = Y + (size_t)pm * (K + kl) * ng;
+  cp_gi x[K];
+  uint64_t fr[K], fi[K];
+  const double two_m64 = 0x1p-64;
+  double acc_r = 0.0, acc_i = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    x[j] = y[(size_t)j * ng + c];
+    uint64_t lr, li;
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, fr[j]);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, fi[j]);
+    acc_r = fma((double)lr, two_m64, acc_r);
+    acc_i = fma((double)li, two_m64, acc_i);
+  }
+  const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
+  const uint64_t b = H->b[i].p;
+  uint64_t rr = 0, ri = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    rr = addmod_ref(rr, addmod_ref(shoup2(x[j].re, H->wint[i][j], b), fr[j], b), b);
+    ri = addmod_ref(ri, addmod_ref(shoup2(x[j].imag, H->wint[i][j], b), fi[j], b), b);
+  }
+  rr = addmod_ref(rr, round_r, b);
+  ri = addmod_ref(ri, round_i, b);
+  const cp_gi yv = y[(size_t)(K + i) * ng + c];
+  rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
+  ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+  Z[((size_t)pm * kl + i) * ng + c] = cp_gi{shoup2(rr, H->invBbi[i], b), shoup2(ri, H->invBbi[i], b)};
+}
+
+// B -> Q, output residue j = blockIdx.z of output polynomial blockIdx.y
+__global__ void __launch_bounds__(256)
+    k_b_to_q_wide(cp_cts out, const cp_gi *__restrict__ Z, int ng, int k,
+                  const cp_hps *__restrict__ H) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ng) return;
+  const int kl = H->kl, j = blockIdx.z, pm = blockIdx.y;
+  const int pair = pm / 3, m = pm % 3;
+  const uint64_t q = H->q[j].p;
+  const cp_gi *z = Z + (size_t)pm * kl * ng;
+  double vr = 0.0, vi = 0.0;
+  uint64_t sr = 0, si = 0;
+  for (int i = 0; i < kl; i++) {  // ascending i, as the per-slot form
+    const cp_gi a = z[(size_t)i * ng + c];
+    const double bd = (double)H->b[i].p;
+    vr = vr + (double)a.re / bd;
+    vi = vi + (double)a.imag / bd;
+    sr = addmod_ref(sr, shoup2(a.re, H->Bbiqi[j][i], q), q);
+    si = addmod_ref(si, shoup2(a.imag, H->Bbiqi[j][i], q), q);
+  }
+  const uint64_t v_re = snap(vr), v_im = snap(vi);
+  cp_gi r;
+  r.re = submod_ref(sr, shoup2(v_re, H->Bqi[j], q), q);
+  r.imag = submod_ref(si, shoup2(v_im, H->Bqi[j], q), q);
+  out.at(pair)[((size_t)m * k + j) * ng + c] = r;
+}
+
 // Tensor product fused into the inverse transform (n <= 2^14): one
 // workgroup row per output residue polynomial (pair, m, r) reads the
 // thread-major spectra X[pair][0..3][r] written by k_dgt_fwd_tm, forms
@@ -305,6 +425,38 @@ static void launch_scale(cupoly_ctx *c, cp_cts out, const cp_gi *Y, int batch) {
                      out, Y, ng, c->e->d_hps);
 }
 
+template <int K>
+static void launch_scale_wide(cupoly_ctx *c, cp_cts out, const cp_gi *Y, cp_gi *Z, int batch) {
+  const int ng = c->e->ng, kl = c->e->kl;
+  const dim3 g((ng + 255) / 256, batch * 3);
+  hipLaunchKernelGGL(k_scale_b_wide<K>, dim3(g.x, g.y, kl), dim3(256), 0, c->stream, Z, Y, ng,
+                     c->e->d_hps);
+  hipLaunchKernelGGL(k_b_to_q_wide, dim3(g.x, g.y, K), dim3(256), 0, c->stream, out, Z, ng, K,
+                     c->e->d_hps);
+}
+
+static int scale_to_q_wide(cupoly_ctx *c, cp_cts out, const cp_gi *Y, cp_gi *Z, int batch) {
+  switch (c->e->k) {
+    case 1: launch_scale_wide<1>(c, out, Y, Z, batch); break;
+    case 2: launch_scale_wide<2>(c, out, Y, Z, batch); break;
+    case 3: launch_scale_wide<3>(c, out, Y, Z, batch); break;
+    case 4: launch_scale_wide<4>(c, out, Y, Z, batch); break;
+    case 5: launch_scale_wide<5>(c, out, Y, Z, batch); break;
+    case 6: launch_scale_wide<6>(c, out, Y, Z, batch); break;
+    case 7: launch_scale_wide<7>(c, out, Y, Z, batch); break;
+    case 8: launch_scale_wide<8>(c, out, Y, Z, batch); break;
+    case 9: launch_scale_wide<9>(c, out, Y, Z, batch); break;
+    case 10: launch_scale_wide<10>(c, out, Y, Z, batch); break;
+    case 11: launch_scale_wide<11>(c, out, Y, Z, batch); break;
+    case 12: launch_scale_wide<12>(c, out, Y, Z, batch); break;
+    case 13: launch_scale_wide<13>(c, out, Y, Z, batch); break;
+    case 14: launch_scale_wide<14>(c, out, Y, Z, batch); break;
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "hom_mul: k outside 1..14");
+  }
+  CP_LAUNCH_CHECK();
+  return CUPOLY_OK;
+}
+
 static int scale_to_q(cupoly_ctx *c, cp_cts out, const cp_gi *Y, int batch) {
   switch (c->e->k) {
     case 1: launch_scale<1>(c, out, Y, batch); break;
@@ -383,8 +535,14 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
   const bool fused_sizes = e->logng >= 10 && e->logng <= 13;
-  hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1, ct2,
-                     ng, e->d_hps, !fused_sizes);
+  // per-residue threads while the per-slot forms would leave the GPU mostly idle
+  const bool wide = (long)batch * 3 * ng < c->wide_slots;
+  if (wide)
+    hipLaunchKernelGGL(k_ext_q_to_b_wide, dim3(sg.x, batch * 4, kl), dim3(256), 0, c->stream, X,
+                       ct1, ct2, ng, e->d_hps, !fused_sizes);
+  else
+    hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1,
+                       ct2, ng, e->d_hps, !fused_sizes);
   CP_LAUNCH_CHECK();
   if (fused_sizes) {
     if ((rc = fwd_tm(c, X, batch * 4 * nres, ct1, ct2))) return rc;
@@ -399,7 +557,8 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     CP_LAUNCH_CHECK();
     if ((rc = cp_dgt_launch(c, Y, 0, nres, batch * 3, CUPOLY_INVERSE))) return rc;
   }
-  return scale_to_q(c, out, Y, batch);
+  // X is dead once the products are formed: it holds the B -> Q operands
+  return wide ? scale_to_q_wide(c, out, Y, X, batch) : scale_to_q(c, out, Y, batch);
 }
 
 extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,

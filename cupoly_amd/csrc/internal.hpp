@@ -84,6 +84,9 @@ struct cupoly_ctx {
   // 0 = the built-in choice
   int tune_rows = 0;        // residue polynomials per workgroup, n = 2^11 .. 2^13
   int tune_persistent = 0;  // n = 2^15: 1 one workgroup per product, -1 two-kernel split
+  // hom-mul: below this many output slots (batch * 3 * n/2) the HPS steps run
+  // one thread per (slot, residue) instead of one per slot
+  long wide_slots = 1L << 16;
 };
 
 // error plumbing
