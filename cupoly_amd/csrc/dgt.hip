@@ -585,6 +585,8 @@ extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupo
                                        int *persistent) {
   if (!c || !out || !a || !b || batch < 1 || reps < 1)
     return cp_fail(CUPOLY_ERR_ARG, "polymul_autotune: bad argument");
+  if (out == a || out == b)  // every candidate runs on the same operands
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_autotune: out must not be a or b");
   // candidate (rows, persistent) pairs for this ring degree
   std::vector<std::pair<int, int>> cand;
   const int logng = c->e->logng;

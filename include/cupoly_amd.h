@@ -191,8 +191,9 @@ int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, cons
 int cupoly_polymul_set_tuning(cupoly_ctx *c, int rows, int persistent);
 /* Autotuner for the batched product: times every available (rows,
  * persistent) choice for this ring degree over `reps` launches on the
- * caller's operands, keeps the fastest on the context, reports it in *rows /
- * *persistent (either may be NULL) and leaves the product in out. */
+ * caller's operands (out must not be a or b), keeps the fastest on the
+ * context, reports it in *rows / *persistent (either may be NULL) and leaves
+ * the product in out. */
 int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
                             int batch, int reps, int *rows, int *persistent);
 /* The same products one launch per op, in the reference's order
