@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -220,6 +222,18 @@ inline void poly_copy(Context *ctx, poly_t *b, poly_t *a) {
   b->status = a->status;
 }
 
+// poly_copy_to_device / poly_copy_to_host (polynomial.cu:105-188) bring the
+// reference's host ZZ coefficients and its device residues into step.  This
+// poly_t holds only the device residues, which are always current; host
+// values are read with poly_get_residues / poly_get_integers.  Both calls
+// only check that the polynomial exists.
+inline void poly_copy_to_device(Context *, poly_t *a) {
+  if (!a || !a->init) throw std::runtime_error("poly_copy_to_device: polynomial not initialised");
+}
+inline void poly_copy_to_host(Context *, poly_t *a) {
+  if (!a || !a->init) throw std::runtime_error("poly_copy_to_host: polynomial not initialised");
+}
+
 inline void poly_add(Context *ctx, poly_t *c, poly_t *a, poly_t *b) {
   poly_init(ctx, c, a->base);
   cupoly_detail::check(cupoly_add_dgt(ctx->ctx, c->d_coefs, a->d_coefs, b->d_coefs, a->base));
@@ -327,6 +341,36 @@ inline void poly_complex_scaling_tDivQ(Context *ctx, poly_t *b, poly_t *a_Q, pol
   b->status = RNSSTATE;
 }
 
+// poly_basis_extension_Q_to_QB (polynomial.cu:459-527): b = a's Q residues
+// followed by its extension to B, both in the DGT domain.
+inline void poly_basis_extension_Q_to_QB(Context *ctx, poly_t *b, poly_t *a) {
+  if (a->base != QBase) throw std::runtime_error("poly_basis_extension_Q_to_QB: a must be in base Q");
+  poly_init(ctx, b, QBBase);
+  if (b->base != QBBase) throw std::runtime_error("poly_basis_extension_Q_to_QB: b must be in base QB");
+  const size_t qoff = (size_t)CUDAEngine::get_n_residues(QBase) * CUDAEngine::N;
+  cupoly_detail::check(cupoly_poly_basis_ext_q_to_b(ctx->ctx, b->d_coefs + qoff, a->d_coefs));
+  cupoly_detail::hip_check(hipMemcpyAsync(b->d_coefs, a->d_coefs, poly_get_residues_size(QBase),
+                                          hipMemcpyDeviceToDevice, ctx->get_stream()),
+                           "poly_basis_extension_Q_to_QB");
+  b->status = RNSSTATE;
+}
+
+// poly_simple_scaling_tDivQ (polynomial.cu:310-346): b = round(t a / Q) mod t
+// in base T, coefficient domain (a transform over base T is the identity, so
+// poly_get_residues reads the 2N values back as they are).  a is unchanged.
+inline void poly_simple_scaling_tDivQ(Context *ctx, poly_t *b, poly_t *a) {
+  if (a->base != QBase) throw std::runtime_error("poly_simple_scaling_tDivQ: a must be in base Q");
+  poly_init(ctx, b, QBase);
+  if (b->d_coefs != a->d_coefs)
+    cupoly_detail::hip_check(hipMemcpyAsync(b->d_coefs, a->d_coefs, poly_get_residues_size(QBase),
+                                            hipMemcpyDeviceToDevice, ctx->get_stream()),
+                             "poly_simple_scaling_tDivQ");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, b->d_coefs, QBase, INVERSE));
+  cupoly_detail::check(cupoly_simple_scaling(ctx->ctx, b->d_coefs));
+  b->status = RNSSTATE;
+  b->base = TBase;
+}
+
 // Coefficient-domain residues of a, [nres][2N] (re half then imag half per
 // residue), as poly_export_residues lays them out (polynomial.cu:1321-1330).
 inline std::vector<uint64_t> poly_get_residues(Context *ctx, poly_t *a) {
@@ -351,6 +395,30 @@ inline std::vector<uint64_t> poly_get_residues(Context *ctx, poly_t *a) {
       out[(size_t)r * 2 * N + N + c] = h[(size_t)r * N + c].imag;
     }
   return out;
+}
+
+// poly_get_residue (polynomial.cu:1538-1592): the 2N coefficient-domain
+// values of residue id (re half then imag half).  Written to b when b is
+// given, else to a new malloc'd array the caller frees.
+inline uint64_t *poly_get_residue(Context *ctx, uint64_t *b, poly_t *a, int id) {
+  if (id < 0 || id >= CUDAEngine::get_n_residues(a->base))
+    throw std::runtime_error("poly_get_residue: residue id out of range");
+  const std::vector<uint64_t> v = poly_get_residues(ctx, a);
+  const size_t w = (size_t)2 * CUDAEngine::N;
+  uint64_t *r = b ? b : (uint64_t *)malloc(w * sizeof(uint64_t));
+  if (!r) throw std::runtime_error("poly_get_residue: out of host memory");
+  std::copy(v.begin() + (size_t)id * w, v.begin() + (size_t)(id + 1) * w, r);
+  return r;
+}
+
+// poly_residue_to_string (polynomial.cu:1489-1536): residue id as "v0,v1,...,"
+// (every value followed by a comma).
+inline std::string poly_residue_to_string(Context *ctx, poly_t *a, int id) {
+  std::vector<uint64_t> v((size_t)2 * CUDAEngine::N);
+  poly_get_residue(ctx, v.data(), a, id);
+  std::ostringstream os;
+  for (uint64_t x : v) os << x << ",";
+  return os.str();
 }
 
 inline std::string poly_export_residues(Context *ctx, poly_t *a) {
@@ -437,6 +505,55 @@ inline std::vector<uint64_t> poly_get_integers(Context *ctx, poly_t *a) {
 inline bool poly_are_equal(Context *ctx, poly_t *a, poly_t *b) {
   if (a->base != b->base) return false;
   return poly_get_residues(ctx, a) == poly_get_residues(ctx, b);
+}
+
+// poly_select_residue (polynomial.cu:1611-1643): every residue of p becomes
+// residue id read as an integer polynomial.  The values of residue id lie in
+// [0, p_id) and are reduced mod each prime of p's base on the way in, so every
+// residue stays in [0, p_i) when p_i < p_id (the reference copies the raw
+// values over).
+inline void poly_select_residue(Context *ctx, poly_t *p, int id) {
+  const int nres = CUDAEngine::get_n_residues(p->base), N = CUDAEngine::N;
+  if (!p->init || id < 0 || id >= nres)
+    throw std::runtime_error("poly_select_residue: bad argument");
+  if (nres == 1) return;
+  uint64_t *d = nullptr;
+  cupoly_detail::hip_check(hipMalloc((void **)&d, (size_t)2 * N * sizeof(uint64_t)),
+                           "poly_select_residue");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, p->base, INVERSE));
+  // residue id as 2N one-limb integers: coefficient j from re[j], j + N from imag[j]
+  const GaussianInteger *r = p->d_coefs + (size_t)id * N;
+  cupoly_detail::hip_check(hipMemcpy2DAsync(d, sizeof(uint64_t), &r->re, sizeof(GaussianInteger),
+                                            sizeof(uint64_t), N, hipMemcpyDeviceToDevice,
+                                            ctx->get_stream()),
+                           "poly_select_residue");
+  cupoly_detail::hip_check(hipMemcpy2DAsync(d + N, sizeof(uint64_t), &r->imag,
+                                            sizeof(GaussianInteger), sizeof(uint64_t), N,
+                                            hipMemcpyDeviceToDevice, ctx->get_stream()),
+                           "poly_select_residue");
+  cupoly_detail::check(cupoly_crt(ctx->ctx, p->d_coefs, d, 2 * N, 1, p->base));
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, p->base, FORWARD));
+  ctx->sync();
+  (void)hipFree(d);
+}
+
+// poly_discard_qbase (polynomial.cu:1645-1660): a QB polynomial keeps only its
+// B residues (moved to the front) and becomes a B polynomial; B is left as is.
+// The move goes one residue at a time in ascending order on the context's
+// stream: residue i lands on [i N, (i + 1) N), which holds no residue that is
+// still to be moved (the single overlapping copy the reference issues has no
+// defined result).
+inline void poly_discard_qbase(Context *ctx, poly_t *p) {
+  if (p->base == BBase) return;
+  if (p->base != QBBase) throw std::runtime_error("poly_discard_qbase: p must be in base QB");
+  const int k = CUDAEngine::get_n_residues(QBase), kl = CUDAEngine::get_n_residues(BBase);
+  const size_t N = CUDAEngine::N;
+  for (int i = 0; i < kl; i++)
+    cupoly_detail::hip_check(hipMemcpyAsync(p->d_coefs + i * N, p->d_coefs + (k + i) * N,
+                                            N * sizeof(GaussianInteger), hipMemcpyDeviceToDevice,
+                                            ctx->get_stream()),
+                             "poly_discard_qbase");
+  p->base = BBase;
 }
 
 // Sampler (sampler.h:49-99), counter-based.

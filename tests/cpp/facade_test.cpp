@@ -5,9 +5,13 @@
 // usage: facade_test <case file>
 // The case file (written by tests/test_gpu_facade.py from the CPU oracle) is
 // raw little-endian uint64: k, kl, n, t, then a, b, a*b, a+b as
-// coefficient-domain residues [k][n] (re half then imag half per residue).
+// coefficient-domain residues [k][n] (re half then imag half per residue),
+// then a extended to base B [kl][n], round(t a / Q) mod t [n], a residue id
+// and residue id of a reduced mod every prime of Q [k][n].
 #include <cstdio>
+#include <algorithm>
 #include <fstream>
+#include <string>
 #include <vector>
 
 #include "../../include/cupoly_amd.hpp"
@@ -23,11 +27,17 @@ int main(int argc, char **argv) {
   const int k = (int)hdr[0], kl = (int)hdr[1], n = (int)hdr[2];
   const uint64_t t = hdr[3];
   const size_t cnt = (size_t)k * n;
-  std::vector<uint64_t> a(cnt), b(cnt), prod(cnt), sum(cnt);
+  const size_t bcnt = (size_t)kl * n;
+  std::vector<uint64_t> a(cnt), b(cnt), prod(cnt), sum(cnt), a_b(bcnt), scaled(n), sel(cnt);
+  uint64_t sid = 0;
   f.read((char *)a.data(), cnt * 8);
   f.read((char *)b.data(), cnt * 8);
   f.read((char *)prod.data(), cnt * 8);
   f.read((char *)sum.data(), cnt * 8);
+  f.read((char *)a_b.data(), bcnt * 8);
+  f.read((char *)scaled.data(), (size_t)n * 8);
+  f.read((char *)&sid, 8);
+  f.read((char *)sel.data(), cnt * 8);
   if (!f) {
     std::fprintf(stderr, "short case file\n");
     return 2;
@@ -65,6 +75,46 @@ int main(int argc, char **argv) {
         delete pi;
         if (!ok) {
           std::fprintf(stderr, "integer round trip mismatch\n");
+          return 1;
+        }
+      }
+      // HPS wrappers: Q -> B, Q -> QB then dropping Q, and t/Q simple scaling
+      {
+        poly_t xb, xqb, ts;
+        poly_basis_extension_Q_to_B(&ctx, &xb, pa);
+        poly_basis_extension_Q_to_QB(&ctx, &xqb, pa);
+        std::vector<uint64_t> qb = poly_get_residues(&ctx, &xqb);
+        std::vector<uint64_t> want_qb(a);
+        want_qb.insert(want_qb.end(), a_b.begin(), a_b.end());
+        bool ok = poly_get_residues(&ctx, &xb) == a_b && qb == want_qb;
+        poly_discard_qbase(&ctx, &xqb);
+        ok = ok && xqb.base == BBase && poly_get_residues(&ctx, &xqb) == a_b;
+        poly_simple_scaling_tDivQ(&ctx, &ts, pa);
+        ok = ok && ts.base == TBase && poly_get_residues(&ctx, &ts) == scaled;
+        ok = ok && poly_get_residues(&ctx, pa) == a;  // the input is left as it was
+        poly_free(&ctx, &xb);
+        poly_free(&ctx, &xqb);
+        poly_free(&ctx, &ts);
+        if (!ok) {
+          std::fprintf(stderr, "basis extension / discard / simple scaling mismatch\n");
+          return 1;
+        }
+      }
+      // one residue read back, and one residue copied over all the others
+      {
+        std::vector<uint64_t> r((size_t)n);  // 2N values
+        poly_get_residue(&ctx, r.data(), pa, (int)sid);
+        bool ok = std::equal(r.begin(), r.end(), a.begin() + sid * n);
+        const std::string rs = poly_residue_to_string(&ctx, pa, (int)sid);
+        ok = ok && rs.size() > 0 && rs.back() == ',' &&
+             rs.substr(0, rs.find(',')) == std::to_string(a[sid * n]);
+        poly_t x;
+        poly_copy(&ctx, &x, pa);
+        poly_select_residue(&ctx, &x, (int)sid);
+        ok = ok && poly_get_residues(&ctx, &x) == sel;
+        poly_free(&ctx, &x);
+        if (!ok) {
+          std::fprintf(stderr, "get_residue / select_residue mismatch\n");
           return 1;
         }
       }
