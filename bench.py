@@ -211,8 +211,8 @@ def main():
                          "copy kernel of known size for HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path has no multi-GPU "
-              "form here); run with --gpus 1", file=sys.stderr)
+        print("bench.py: cupoly_amd runs on one GPU only (the path has no multi-GPU "
+              "form); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
