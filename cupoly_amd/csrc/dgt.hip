@@ -563,6 +563,8 @@ extern "C" int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
   // an empty batch is a no-op (its operand pointers may be null)
   if (c && batch == 0) return CUPOLY_OK;
   if (!c || !out || !a || !b || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "polymul_batch: bad argument");
+  if ((long long)batch * c->e->k >= (1LL << 31))
+    return cp_fail(CUPOLY_ERR_ARG, "polymul_batch: batch * k must stay below 2^31");
   return cp_polymul_fused_launch(c, (cp_gi *)out, (const cp_gi *)a, (const cp_gi *)b, batch);
 }
 

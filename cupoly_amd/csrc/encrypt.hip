@@ -111,12 +111,6 @@ static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch,
   const int nres = e->k, nunits = nres * batch;
   constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
-  static bool attr_set = false;
-  if (!attr_set && lds > 64 * 1024) {
-    CP_HIP(hipFuncSetAttribute((const void *)k_sample_muladd<LOGN, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
   // packed samples of every instance: 4 bytes per Gaussian slot
   const long total = (long)batch * e->ng;
   void *ws;
@@ -134,10 +128,40 @@ static int sample_muladd_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, int batch,
   return CUPOLY_OK;
 }
 
+// Dynamic LDS limit of k_sample_muladd, set once per engine on the engine's
+// device (engine creation).
+template <int LOGN>
+static int encrypt_setup_t() {
+  constexpr int R = rows_for<LOGN>();
+  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
+  if (lds > 64 * 1024)
+    CP_HIP(hipFuncSetAttribute((const void *)k_sample_muladd<LOGN, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return CUPOLY_OK;
+}
+
+int cp_encrypt_setup(cupoly_engine *e) {
+  switch (e->logng) {
+    case 4: return encrypt_setup_t<4>();
+    case 5: return encrypt_setup_t<5>();
+    case 6: return encrypt_setup_t<6>();
+    case 7: return encrypt_setup_t<7>();
+    case 8: return encrypt_setup_t<8>();
+    case 9: return encrypt_setup_t<9>();
+    case 10: return encrypt_setup_t<10>();
+    case 11: return encrypt_setup_t<11>();
+    case 12: return encrypt_setup_t<12>();
+    case 13: return encrypt_setup_t<13>();
+    default: return CUPOLY_OK;  // larger rings compose the generic kernels
+  }
+}
+
 extern "C" int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a,
                                           int batch, uint64_t seed, uint64_t stream0) {
   if (c && batch == 0) return CUPOLY_OK;  // empty batch: pointers may be null
   if (!c || !out || !a || batch < 0) return cp_fail(CUPOLY_ERR_ARG, "sample_muladd: bad argument");
+  if ((long long)batch * c->e->k >= (1LL << 31))
+    return cp_fail(CUPOLY_ERR_ARG, "sample_muladd: batch * k must stay below 2^31");
   switch (c->e->logng) {
     case 4: return sample_muladd_t<4>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);
     case 5: return sample_muladd_t<5>(c, (cp_gi *)out, (const cp_gi *)a, batch, seed, stream0);

@@ -367,12 +367,6 @@ template <int LOGN>
 static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
-  static bool attr = false;
-  if (!attr) {
-    CP_HIP(hipFuncSetAttribute((const void *)k_dgt_fwd_tm<LOGN, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
   cupoly_engine *e = c->e;
   hipLaunchKernelGGL((k_dgt_fwd_tm<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
                      c->stream, X, nunits, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
@@ -395,12 +389,6 @@ template <int LOGN>
 static int launch_tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
-  static bool attr = false;
-  if (!attr) {
-    CP_HIP(hipFuncSetAttribute((const void *)k_tensor_idgt<LOGN, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
   cupoly_engine *e = c->e;
   hipLaunchKernelGGL((k_tensor_idgt<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
                      c->stream, Y, X, nunits, e->nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
@@ -415,6 +403,29 @@ static int tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
     case 12: return launch_tensor_idgt<12>(c, Y, X, nunits);
     case 13: return launch_tensor_idgt<13>(c, Y, X, nunits);
     default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "tensor_idgt: ring degree");
+  }
+}
+
+// Dynamic LDS limits of the pipeline's transform kernels, set once per engine
+// on the engine's device (engine creation).
+template <int LOGN>
+static int homomul_setup_t() {
+  constexpr int R = rows_for<LOGN>();
+  const int lds = (int)((size_t)R * lds_row<LOGN>() * sizeof(cp_gi));
+  CP_HIP(hipFuncSetAttribute((const void *)k_dgt_fwd_tm<LOGN, R>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  CP_HIP(hipFuncSetAttribute((const void *)k_tensor_idgt<LOGN, R>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  return CUPOLY_OK;
+}
+
+int cp_homomul_setup(cupoly_engine *e) {
+  switch (e->logng) {
+    case 10: return homomul_setup_t<10>();
+    case 11: return homomul_setup_t<11>();
+    case 12: return homomul_setup_t<12>();
+    case 13: return homomul_setup_t<13>();
+    default: return CUPOLY_OK;  // other sizes use the generic transforms
   }
 }
 

@@ -106,5 +106,7 @@ bool cp_base_range(const cupoly_engine *e, int base, int *off, int *cnt);
 int cp_dgt_launch(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int direction);
 int cp_polymul_fused_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int batch);
 int cp_dgt_setup(cupoly_engine *e);
+int cp_homomul_setup(cupoly_engine *e);
+int cp_encrypt_setup(cupoly_engine *e);
 int cp_workspace(cupoly_ctx *c, size_t bytes, void **out);
 int cp_pointwise_mul_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nres);

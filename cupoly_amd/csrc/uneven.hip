@@ -339,15 +339,12 @@ extern "C" int cupoly_uneven_create(cupoly_ctx *c, cupoly_engine *const *engines
     cupoly_uneven_destroy(u);
     return rc;
   }
-  static bool attr = false;
-  if (!attr) {
-    err = hipFuncSetAttribute((const void *)k_uneven, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              UNEVEN_LDS_SLOTS * sizeof(cp_gi));
-    if (err != hipSuccess) {
-      cupoly_uneven_destroy(u);
-      return cp_check_hip(err, "uneven_create: LDS attribute");
-    }
-    attr = true;
+  // on the current device, for every plan (a plan may be built on any device)
+  err = hipFuncSetAttribute((const void *)k_uneven, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            UNEVEN_LDS_SLOTS * sizeof(cp_gi));
+  if (err != hipSuccess) {
+    cupoly_uneven_destroy(u);
+    return cp_check_hip(err, "uneven_create: LDS attribute");
   }
   *plan = u;
   return CUPOLY_OK;

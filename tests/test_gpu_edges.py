@@ -143,3 +143,16 @@ def test_product_in_place(gpu, n, k, batch):
     assert torch.equal(db, want)
     i = batch - 1
     assert np.array_equal(cp.to_host(want)[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
+
+
+def test_oversized_batch_is_rejected_before_any_launch(gpu):
+    """batch * k at or above 2^31 is an argument error (it used to wrap the
+    unit count), and no kernel runs: the operand buffers are far too small."""
+    e, ctx = engine(2, 2048)
+    t = cp.empty((1, 2, 1024, 2))
+    p = cp._ptr(t)
+    assert cp.lib().cupoly_polymul_batch(ctx.handle, p, p, p, 1 << 30) < 0
+    assert "2^31" in cp.lib().cupoly_last_error().decode()
+    assert cp.lib().cupoly_sample_muladd_batch(ctx.handle, p, p, 1 << 30, 1, 0) < 0
+    assert "2^31" in cp.lib().cupoly_last_error().decode()
+    ctx.sync()
