@@ -161,11 +161,15 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
   cp_gi *lds = lds_all + row * lds_row<LOGN>();
-  int unit = blockIdx.x * ROWS + row;
-  const bool active = unit < nunits;
-  if (!active) unit = nunits - 1;
-  int rid = unit % nres;
+  // workgroups run residue-major (all units of residue 0, then residue 1, ...)
+  // so that consecutive workgroups share one residue's tables in L2
+  int u = blockIdx.x * ROWS + row;
+  const bool active = u < nunits;
+  if (!active) u = nunits - 1;
+  const int per_r = nunits / nres;
+  int rid = u / per_r;
   if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
+  const int unit = (u - rid * per_r) * nres + rid;
   const uint64_t p = primes[rid].p;
   const uint64_t *tw = twf_all + (size_t)rid * N, *tt = tsf_all + (size_t)rid * N * 6;
   cp_gi *poly = X + (size_t)unit * N;
@@ -327,12 +331,15 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
   cp_gi *lds = lds_all + row * lds_row<LOGN>();
-  int unit = blockIdx.x * ROWS + row;  // (pair * 3 + m) * nres + r
-  const bool active = unit < nunits;
-  if (!active) unit = nunits - 1;
-  int rid = unit % nres;
+  // residue-major workgroup order, as in k_dgt_fwd_tm
+  int u = blockIdx.x * ROWS + row;
+  const bool active = u < nunits;
+  if (!active) u = nunits - 1;
+  const int per_r = nunits / nres;
+  int rid = u / per_r;
   if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
-  const int pm = unit / nres, pair = pm / 3, m = pm % 3;
+  const int pm = u - rid * per_r, unit = pm * nres + rid;  // (pair * 3 + m) * nres + r
+  const int pair = pm / 3, m = pm % 3;
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
   const size_t per_pair = (size_t)nres * N;
