@@ -103,9 +103,13 @@ __global__ void __launch_bounds__(256)
     acc_i = fma((double)li, two_m64, acc_i);
   }
   const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
-  uint64_t sr[K], si[K];
+  // B -> Q sums, exact (every term is reduced) and reduced once per output
+  cp_acc sr[K], si[K];
 #pragma unroll
-  for (int j = 0; j < K; j++) sr[j] = si[j] = 0;
+  for (int j = 0; j < K; j++) {
+    cp_acc_init(sr[j]);
+    cp_acc_init(si[j]);
+  }
   double vr = 0.0, vi = 0.0;
   for (int i = 0; i < kl; i++) {
     const uint64_t b = H->b[i].p;
@@ -127,19 +131,21 @@ __global__ void __launch_bounds__(256)
     vi = vi + (double)ai / bd;
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      const uint64_t q = H->q[j].p;
-      sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
-      si[j] = addmod_ref(si[j], shoup2(ai, H->Bbiqi[j][i], q), q);
+      cp_acc_mac(sr[j], ar, H->Bbiqi[j][i][0]);
+      cp_acc_mac(si[j], ai, H->Bbiqi[j][i][0]);
     }
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   cp_gi *o = out.at(pair) + (size_t)m * K * ng;
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const uint64_t q = H->q[j].p;
+    const uint64_t q = H->q[j].p, bq = H->Bqi[j][0];
+    const uint64_t nbq = bq ? q - bq : 0;  // -B mod q_j
+    cp_acc_mac(sr[j], v_re, nbq);
+    cp_acc_mac(si[j], v_im, nbq);
     cp_gi r;
-    r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
-    r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
+    r.re = cp_acc_reduce(sr[j], q);
+    r.imag = cp_acc_reduce(si[j], q);
     o[(size_t)j * ng + c] = r;
   }
 }
@@ -236,18 +242,22 @@ __global__ void __launch_bounds__(256)
     }
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
-  const uint64_t b = H->b[i].p;
-  uint64_t sr = 0, si = 0;
+  // exact sum, reduced once (as hps_q_to_b_slot)
+  const uint64_t b = H->b[i].p, qb = H->Qbi[i][0];
+  const uint64_t nqb = qb ? b - qb : 0;  // -Q mod b_i
+  cp_acc sr, si;
+  cp_acc_init(sr);
+  cp_acc_init(si);
 #pragma unroll
   for (int j = 0; j < CP_MAX_BASE; j++) {
     if (j < k) {
-      sr = addmod_ref(sr, shoup2(ar[j], H->Qqibi[i][j], b), b);
-      si = addmod_ref(si, shoup2(ai[j], H->Qqibi[i][j], b), b);
+      cp_acc_mac(sr, ar[j], H->Qqibi[i][j][0]);
+      cp_acc_mac(si, ai[j], H->Qqibi[i][j][0]);
     }
   }
-  sr = submod_ref(sr, shoup2(v_re, H->Qbi[i], b), b);
-  si = submod_ref(si, shoup2(v_im, H->Qbi[i], b), b);
-  out[(size_t)(k + i) * ng + c] = cp_gi{sr, si};
+  cp_acc_mac(sr, v_re, nqb);
+  cp_acc_mac(si, v_im, nqb);
+  out[(size_t)(k + i) * ng + c] = cp_gi{cp_acc_reduce(sr, b), cp_acc_reduce(si, b)};
 }
 
 // t/Q scaling, B residue i = blockIdx.z of output polynomial blockIdx.y, and
@@ -300,19 +310,24 @@ __global__ void __launch_bounds__(256)
   const uint64_t q = H->q[j].p;
   const cp_gi *z = Z + (size_t)pm * kl * ng;
   double vr = 0.0, vi = 0.0;
-  uint64_t sr = 0, si = 0;
+  cp_acc sr, si;  // exact sums, reduced once (as hps_b_to_q_slot)
+  cp_acc_init(sr);
+  cp_acc_init(si);
   for (int i = 0; i < kl; i++) {  // ascending i, as the per-slot form
     const cp_gi a = z[(size_t)i * ng + c];
     const double bd = (double)H->b[i].p;
     vr = vr + (double)a.re / bd;
     vi = vi + (double)a.imag / bd;
-    sr = addmod_ref(sr, shoup2(a.re, H->Bbiqi[j][i], q), q);
-    si = addmod_ref(si, shoup2(a.imag, H->Bbiqi[j][i], q), q);
+    cp_acc_mac(sr, a.re, H->Bbiqi[j][i][0]);
+    cp_acc_mac(si, a.imag, H->Bbiqi[j][i][0]);
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
+  const uint64_t bq = H->Bqi[j][0], nbq = bq ? q - bq : 0;  // -B mod q_j
+  cp_acc_mac(sr, v_re, nbq);
+  cp_acc_mac(si, v_im, nbq);
   cp_gi r;
-  r.re = submod_ref(sr, shoup2(v_re, H->Bqi[j], q), q);
-  r.imag = submod_ref(si, shoup2(v_im, H->Bqi[j], q), q);
+  r.re = cp_acc_reduce(sr, q);
+  r.imag = cp_acc_reduce(si, q);
   out.at(pair)[((size_t)m * k + j) * ng + c] = r;
 }
 

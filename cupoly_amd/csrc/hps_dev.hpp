@@ -70,20 +70,26 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   for (int i = 0; i < kl; i++) {
-    const uint64_t b = H->b[i].p;
-    uint64_t sr = 0, si = 0;
+    // (sum_j a_j (Q/q_j mod b_i) - v (Q mod b_i)) mod b_i: every term is
+    // reduced, so the exact sum reduced once is the reference's chain of
+    // addmod / submod
+    const uint64_t b = H->b[i].p, qb = H->Qbi[i][0];
+    const uint64_t nqb = qb ? b - qb : 0;  // -Q mod b_i
+    cp_acc sr, si;
+    cp_acc_init(sr);
+    cp_acc_init(si);
 #pragma unroll
     for (int j = 0; j < CP_MAX_BASE; j++) {
       if (j < k) {
-        sr = addmod_ref(sr, shoup2(ar[j], H->Qqibi[i][j], b), b);
-        si = addmod_ref(si, shoup2(ai[j], H->Qqibi[i][j], b), b);
+        cp_acc_mac(sr, ar[j], H->Qqibi[i][j][0]);
+        cp_acc_mac(si, ai[j], H->Qqibi[i][j][0]);
       }
     }
-    sr = submod_ref(sr, shoup2(v_re, H->Qbi[i], b), b);
-    si = submod_ref(si, shoup2(v_im, H->Qbi[i], b), b);
+    cp_acc_mac(sr, v_re, nqb);
+    cp_acc_mac(si, v_im, nqb);
     cp_gi o;
-    o.re = sr;
-    o.imag = si;
+    o.re = cp_acc_reduce(sr, b);
+    o.imag = cp_acc_reduce(si, b);
     out[(size_t)i * bs + c] = o;
   }
 }
@@ -161,20 +167,24 @@ __device__ __forceinline__ void hps_b_to_q_slot(const uint64_t (&xr)[CP_MAX_BASE
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   for (int j = 0; j < k; j++) {
-    const uint64_t q = H->q[j].p;
-    uint64_t sr = 0, si = 0;
+    // (sum_i a_i (B/b_i mod q_j) - v (B mod q_j)) mod q_j, reduced once
+    const uint64_t q = H->q[j].p, bq = H->Bqi[j][0];
+    const uint64_t nbq = bq ? q - bq : 0;  // -B mod q_j
+    cp_acc sr, si;
+    cp_acc_init(sr);
+    cp_acc_init(si);
 #pragma unroll
     for (int i = 0; i < CP_MAX_BASE; i++) {
       if (i < kl) {
-        sr = addmod_ref(sr, shoup2(ar[i], H->Bbiqi[j][i], q), q);
-        si = addmod_ref(si, shoup2(ai[i], H->Bbiqi[j][i], q), q);
+        cp_acc_mac(sr, ar[i], H->Bbiqi[j][i][0]);
+        cp_acc_mac(si, ai[i], H->Bbiqi[j][i][0]);
       }
     }
-    sr = submod_ref(sr, shoup2(v_re, H->Bqi[j], q), q);
-    si = submod_ref(si, shoup2(v_im, H->Bqi[j], q), q);
+    cp_acc_mac(sr, v_re, nbq);
+    cp_acc_mac(si, v_im, nbq);
     cp_gi o;
-    o.re = sr;
-    o.imag = si;
+    o.re = cp_acc_reduce(sr, q);
+    o.imag = cp_acc_reduce(si, q);
     out[(size_t)j * qs + c] = o;
   }
 }

@@ -101,6 +101,49 @@ CP_HD uint64_t cp_mulmod(uint64_t a, uint64_t b, const cp_prime& pr) {
   return cp_red2(z, p);
 }
 
+// Exact sums of products of 64-bit words, reduced mod p once at the end
+// (p = 2^63 - delta, delta < 2^30).  The sum is kept in 160 bits,
+// x = top 2^128 + hi 2^64 + lo (up to 2^32 products), and folded with
+// 2^64 = 2 delta and 2^128 = 4 delta^2 (mod p):
+//   y = lo + hi (2 delta) + top (4 delta^2) < 2^64 + 2^94 + 2^92 < 2^95,
+//   z = (y mod 2^63) + (y >> 63) delta < 2^63 + 2^62 < 2p, then one red2.
+// A product costs one 64x64 -> 128 multiply and a three-word add, where a
+// modular chain costs a Shoup product and a conditional correction per term.
+struct cp_acc {
+  uint64_t lo, hi;
+  uint32_t top;
+};
+
+CP_HD uint64_t cp_addc64(uint64_t a, uint64_t b, unsigned &carry) {
+  const uint64_t s = a + b;
+  const unsigned c1 = s < a;
+  const uint64_t r = s + carry;
+  carry = c1 | (unsigned)(r < s);
+  return r;
+}
+
+CP_HD void cp_acc_init(cp_acc &s) {
+  s.lo = 0;
+  s.hi = 0;
+  s.top = 0;
+}
+
+CP_HD void cp_acc_mac(cp_acc &s, uint64_t a, uint64_t b) {
+  const cp_u128 x = (cp_u128)a * b;
+  unsigned c = 0;
+  s.lo = cp_addc64(s.lo, (uint64_t)x, c);
+  s.hi = cp_addc64(s.hi, (uint64_t)(x >> 64), c);
+  s.top += c;
+}
+
+CP_HD uint64_t cp_acc_reduce(const cp_acc &s, uint64_t p) {
+  const uint64_t delta = (1ULL << 63) - p;
+  const cp_u128 y = (cp_u128)s.lo + (cp_u128)s.hi * (2 * delta) +
+                    (cp_u128)s.top * (4 * delta * delta);
+  const uint64_t z = ((uint64_t)y & ((1ULL << 63) - 1)) + (uint64_t)(y >> 63) * delta;
+  return cp_red2(z, p);
+}
+
 // Gaussian integer product (a.re + i a.im)(b.re + i b.im) mod p, Karatsuba
 // with three exact products as the reference's GIMul.
 CP_HD cp_gi cp_gimul(cp_gi a, cp_gi b, const cp_prime& pr) {

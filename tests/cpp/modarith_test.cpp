@@ -54,6 +54,24 @@ int main(int argc, char **argv) {
         std::printf("red2 FAIL\n");
         return 1;
       }
+      // exact sums of 1..40 products of any 64-bit words (all-ones words in
+      // the edge iterations, so that the top word carries)
+      if (it % 16 == 0) {
+        cp_acc acc;
+        cp_acc_init(acc);
+        uint64_t want4 = 0;
+        const int terms = 1 + (int)(next64() % 40);
+        for (int j = 0; j < terms; j++) {
+          const uint64_t x = it < 144 ? ~0ULL : next64(), y = it < 144 ? ~0ULL : next64();
+          cp_acc_mac(acc, x, y);
+          want4 = (uint64_t)(((cp_u128)want4 + (uint64_t)(((cp_u128)x * y) % p)) % p);
+        }
+        if (cp_acc_reduce(acc, p) != want4) {
+          std::printf("acc FAIL p=%llu terms=%d\n", (unsigned long long)p, terms);
+          return 1;
+        }
+        checks++;
+      }
       checks += 4;
     }
   }
