@@ -174,6 +174,36 @@ def test_hps_kernels_match_oracle(gpu, k, n, t):
     assert np.array_equal(cp.to_host(d)[0], O.simple_scaling(P, xq))
 
 
+@pytest.mark.parametrize("k", [8, 13])
+def test_hps_base_changes_at_extreme_residues(gpu, k):
+    """Q -> B and B -> Q with every residue at its largest value (p - 1) in
+    half of the slots and 0 in a quarter, up to k = 13, kl = 14 (the largest
+    bases): the largest sums the 160-bit accumulation of the base changes
+    sees, against the oracle's modular chains."""
+    kl, n, t = k + 1, 1024, 65537
+    e, ctx = engine(k, kl, n, t)
+    P = O.Params(k, kl, n, t)
+    half, quarter = P.Ng // 2, P.Ng // 4
+    xq = rand_res(P, P.q, (), 70 + k)
+    for r, q in enumerate(P.q):
+        xq[r, :half] = q - 1
+        xq[r, half:half + quarter] = 0
+    buf = np.zeros((max(k, kl), P.Ng, 2), dtype=np.uint64)
+    buf[:k] = xq
+    d = cp.to_device(buf)
+    cp.execute_polynomial_basis_ext_Q_to_B(ctx, d)
+    assert np.array_equal(cp.to_host(d)[:kl], O.basis_ext_q_to_b(P, xq))
+    rng = np.random.default_rng(80 + k)
+    xb = np.empty((kl, n), dtype=np.uint64)
+    for i, b in enumerate(P.b):
+        xb[i] = rng.integers(0, b, size=n, dtype=np.uint64)
+        xb[i, : n // 2] = b - 1
+        xb[i, n // 2: n // 2 + n // 4] = 0
+    oq = cp.empty((k, P.Ng, 2))
+    cp.execute_polynomial_basis_ext_B_to_Q(ctx, oq, cp.to_device(xb))
+    assert np.array_equal(cp.to_host(oq), O.basis_ext_b_to_q(P, xb))
+
+
 @pytest.mark.parametrize("k,n,t", [(4, 16384, 65537), (3, 4096, 256), (2, 2048, 40961)])
 def test_simple_scaling_recovers_scaled_messages(gpu, k, n, t):
     """Decryption shape: x = floor(Q/t) m + e (|e| small) scaled by t/Q and
