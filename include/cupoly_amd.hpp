@@ -266,6 +266,7 @@ inline void poly_dr2(Context *ctx, poly_t *ct21, poly_t *ct22, poly_t *ct23, pol
   cupoly_detail::check(cupoly_dr2_dgt(ctx->ctx, ct21->d_coefs, ct22->d_coefs, ct23->d_coefs,
                                       ct01->d_coefs, ct02->d_coefs, ct11->d_coefs, ct12->d_coefs,
                                       ct01->base));
+  ct21->status = ct22->status = ct23->status = RNSSTATE;
 }
 // poly_xi_bfv / poly_rho_bfv (polynomial.cu:610-693): c is an array of k
 // polynomials; residue i of c[i] receives the scaled residue i of a.
@@ -303,14 +304,17 @@ inline void poly_dot(Context *ctx, poly_t *c, poly_t *a, poly_t *b, const int k)
 inline void poly_mul_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
   poly_init(ctx, c, a->base);
   cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, MUL, a->base));
+  c->status = RNSSTATE;
 }
 inline void poly_add_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
   poly_init(ctx, c, a->base);
   cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, ADD, a->base));
+  c->status = RNSSTATE;
 }
 inline void poly_sub_int(Context *ctx, poly_t *c, poly_t *a, uint64_t x) {
   poly_init(ctx, c, a->base);
   cupoly_detail::check(cupoly_op_by_int(ctx->ctx, c->d_coefs, a->d_coefs, x, SUB, a->base));
+  c->status = RNSSTATE;
 }
 inline void poly_double_mul_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2, poly_t *a2,
                                 uint64_t x1, uint64_t x2) {
@@ -318,6 +322,7 @@ inline void poly_double_mul_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2
   poly_init(ctx, c2, a2->base);
   cupoly_detail::check(cupoly_double_op_by_int(ctx->ctx, c1->d_coefs, a1->d_coefs, x1,
                                                c2->d_coefs, a2->d_coefs, x2, MULMUL, a1->base));
+  c1->status = c2->status = RNSSTATE;
 }
 inline void poly_double_add_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2, poly_t *a2,
                                 uint64_t x1, uint64_t x2) {
@@ -325,6 +330,7 @@ inline void poly_double_add_int(Context *ctx, poly_t *c1, poly_t *a1, poly_t *c2
   poly_init(ctx, c2, a2->base);
   cupoly_detail::check(cupoly_double_op_by_int(ctx->ctx, c1->d_coefs, a1->d_coefs, x1,
                                                c2->d_coefs, a2->d_coefs, x2, ADDADD, a1->base));
+  c1->status = c2->status = RNSSTATE;
 }
 inline void poly_negate(Context *ctx, poly_t *a) {
   cupoly_detail::check(cupoly_negate(ctx->ctx, a->d_coefs, a->d_coefs, a->base));

@@ -16,6 +16,8 @@
 
 #include "../../include/cupoly_amd.hpp"
 
+typedef unsigned __int128 u128;
+
 int main(int argc, char **argv) {
   if (argc != 2) {
     std::fprintf(stderr, "usage: %s <case file>\n", argv[0]);
@@ -77,6 +79,75 @@ int main(int argc, char **argv) {
           std::fprintf(stderr, "integer round trip mismatch\n");
           return 1;
         }
+      }
+      // slot-wise and scalar wrappers against residue arithmetic done here
+      // (the DGT is linear, and an integer x is the constant polynomial x)
+      {
+        const std::vector<uint64_t> &Q = CUDAEngine::RNSPrimes;
+        const uint64_t x1 = 123456789123456789ULL, x2 = 987654321ULL;
+        auto modq = [&](int r, u128 v) { return (uint64_t)(v % Q[r]); };
+        std::vector<uint64_t> w_sub(cnt), w_neg(cnt), w_mul1(cnt), w_mul2(cnt), w_add1(a),
+            w_sub1(a), w_add2(b), w_bb(cnt), w_muladd(cnt);
+        for (int r = 0; r < k; r++) {
+          const uint64_t q = Q[r], y1 = x1 % q, y2 = x2 % q;
+          for (int j = 0; j < n; j++) {
+            const size_t i = (size_t)r * n + j;
+            w_sub[i] = modq(r, (u128)a[i] + q - b[i]);
+            w_neg[i] = modq(r, (u128)q - a[i]);
+            w_mul1[i] = modq(r, (u128)a[i] * y1);
+            w_mul2[i] = modq(r, (u128)b[i] * y2);
+            w_bb[i] = modq(r, (u128)b[i] + b[i]);
+            w_muladd[i] = modq(r, (u128)prod[i] + a[i]);
+          }
+          const size_t i0 = (size_t)r * n;  // coefficient 0
+          w_add1[i0] = modq(r, (u128)a[i0] + y1);
+          w_sub1[i0] = modq(r, (u128)a[i0] + q - y1);
+          w_add2[i0] = modq(r, (u128)b[i0] + y2);
+        }
+        poly_t s, ng, m1, ad1, sb1, da1, da2, dm1, dm2, di1, di2, ma, cl, cc;
+        poly_sub(&ctx, &s, pa, pb);
+        poly_copy(&ctx, &ng, pa);
+        poly_negate(&ctx, &ng);
+        poly_mul_int(&ctx, &m1, pa, x1);
+        poly_add_int(&ctx, &ad1, pa, x1);
+        poly_sub_int(&ctx, &sb1, pa, x1);
+        poly_double_add(&ctx, &da1, pa, pb, &da2, pb, pb);
+        poly_double_mul_int(&ctx, &dm1, pa, &dm2, pb, x1, x2);
+        poly_double_add_int(&ctx, &di1, pa, &di2, pb, x1, x2);
+        poly_mul_add(&ctx, &ma, pa, pb, pa);
+        poly_copy(&ctx, &cl, pa);
+        poly_clear(&ctx, &cl);
+        poly_copy(&ctx, &cc, pb);
+        struct {
+          poly_t *p;
+          const std::vector<uint64_t> *want;
+          const char *name;
+        } checks[] = {{&s, &w_sub, "poly_sub"},
+                      {&ng, &w_neg, "poly_negate"},
+                      {&m1, &w_mul1, "poly_mul_int"},
+                      {&ad1, &w_add1, "poly_add_int"},
+                      {&sb1, &w_sub1, "poly_sub_int"},
+                      {&da1, &sum, "poly_double_add (first)"},
+                      {&da2, &w_bb, "poly_double_add (second)"},
+                      {&dm1, &w_mul1, "poly_double_mul_int (first)"},
+                      {&dm2, &w_mul2, "poly_double_mul_int (second)"},
+                      {&di1, &w_add1, "poly_double_add_int (first)"},
+                      {&di2, &w_add2, "poly_double_add_int (second)"},
+                      {&ma, &w_muladd, "poly_mul_add"},
+                      {&cc, &b, "poly_copy"}};
+        for (auto &ch : checks)
+          if (poly_get_residues(&ctx, ch.p) != *ch.want) {
+            std::fprintf(stderr, "%s mismatch\n", ch.name);
+            return 1;
+          }
+        for (uint64_t v : poly_get_residues(&ctx, &cl))
+          if (v != 0) {
+            std::fprintf(stderr, "poly_clear left a nonzero residue\n");
+            return 1;
+          }
+        for (poly_t *p : {&s, &ng, &m1, &ad1, &sb1, &da1, &da2, &dm1, &dm2, &di1, &di2, &ma, &cl,
+                          &cc})
+          poly_free(&ctx, p);
       }
       // HPS wrappers: Q -> B, Q -> QB then dropping Q, and t/Q simple scaling
       {
