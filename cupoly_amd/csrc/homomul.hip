@@ -385,6 +385,88 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
 }
 
+// ---- n = 2^15: the two kernels above for 16384-slot residue polynomials --
+// One 1024-thread workgroup per residue polynomial, as k_polymul_16k: stage 0
+// pairs slot g with g + 8192 in registers, and the 8192-slot halves then take
+// stages 1..13 in turn through the LDS.  Spectra are thread-major per half,
+// X[unit][h 8192 + i 1024 + t].  While one half is in the passes the other
+// waits in its own final place: the slots a thread stores to there are slots
+// only that thread has read (first_index<13>(t, i) is t modulo 1024), and it
+// read them before.
+__global__ void __launch_bounds__(1024)
+    k_dgt_fwd_tm16(cp_gi *__restrict__ X, int nunits, int nres, int k, cp_cts ct1, cp_cts ct2,
+                   const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                   const uint64_t *__restrict__ tsf_all) {
+  constexpr int N = 16384, H = 8192;
+  extern __shared__ cp_gi lds[];
+  const int t = (int)threadIdx.x;
+  // residue-major workgroup order, as k_dgt_fwd_tm
+  const int per_r = nunits / nres, u = (int)blockIdx.x;
+  const int rid = __builtin_amdgcn_readfirstlane(u / per_r);
+  const int unit = (u - rid * per_r) * nres + rid;
+  const uint64_t p = primes[rid].p;
+  const CP_GLOBAL uint64_t *twf = (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N);
+  const CP_GLOBAL uint64_t *tsf = (const CP_GLOBAL uint64_t *)(tsf_all + (size_t)rid * N * 6);
+  CP_GLOBAL cp_gi *poly = (CP_GLOBAL cp_gi *)(X + (size_t)unit * N);
+  const CP_GLOBAL cp_gi *src = poly;
+  if (rid < k) {  // a Q residue comes straight from its ciphertext
+    const int pw = unit / nres, pair = pw >> 2, which = pw & 3;
+    src = (const CP_GLOBAL cp_gi *)((which < 2 ? ct1.at(pair) : ct2.at(pair)) +
+                                     (size_t)((which & 1) * k + rid) * N);
+  }
+  cp_gi x[EPT], y[EPT];
+  stage0_16k(x, y, src, t, twf, tsf, p);
+  park_st(poly, 1, t, y);
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
+  park_st(poly, 0, t, x);
+  park_ld(poly, 1, t, x);
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{H});
+  park_st(poly, 1, t, x);
+}
+
+__global__ void __launch_bounds__(1024)
+    k_tensor_idgt16(cp_gi *__restrict__ Y, const cp_gi *__restrict__ X, int nunits, int nres,
+                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twi_all,
+                    const uint64_t *__restrict__ tsi_all) {
+  constexpr int N = 16384, H = 8192;
+  extern __shared__ cp_gi lds[];
+  const int t = (int)threadIdx.x;
+  const int per_r = nunits / nres, u = (int)blockIdx.x;
+  const int rid = __builtin_amdgcn_readfirstlane(u / per_r);
+  const int pm = u - rid * per_r, unit = pm * nres + rid;  // (pair * 3 + m) * nres + r
+  const int pair = pm / 3, m = pm % 3;
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const CP_GLOBAL uint64_t *twi = (const CP_GLOBAL uint64_t *)(twi_all + (size_t)rid * N);
+  const CP_GLOBAL uint64_t *tsi = (const CP_GLOBAL uint64_t *)(tsi_all + (size_t)rid * N * 6);
+  const size_t per_pair = (size_t)nres * N;
+  const CP_GLOBAL cp_gi *x = (const CP_GLOBAL cp_gi *)(X + (size_t)pair * 4 * per_pair + (size_t)rid * N);
+  const CP_GLOBAL cp_gi *xa = x + (m == 2 ? per_pair : 0), *xb = x + (m == 0 ? 2 : 3) * per_pair;
+  CP_GLOBAL cp_gi *y = (CP_GLOBAL cp_gi *)(Y + (size_t)unit * N);
+  cp_gi v[EPT], w[EPT];
+#pragma nounroll
+  for (int h = 0; h < 2; h++) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = h * H + i * 1024 + t;  // thread-major
+      v[i] = cp_gimul(ld_gi(xa + g), ld_gi(xb + g), pr);
+      if (m == 1) {  // + c1 d0
+        const cp_gi z = cp_gimul(ld_gi(x + per_pair + g), ld_gi(x + 2 * per_pair + g), pr);
+        v[i].re = cp_add(v[i].re, z.re, p);
+        v[i].imag = cp_add(v[i].imag, z.imag, p);
+      }
+    }
+    dit_passes<14, 13, 0, 13, 1>(v, lds, t, twi, p, MapBlock{h * H});
+    if (h == 0) park_st(y, 0, t, v);  // half 0 waits in its own final place
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; i++) w[i] = v[i];
+  park_ld(y, 0, t, v);
+  int to = t;
+  asm volatile("" : "+v"(to));
+  istage0_16k(y, v, w, to, twi, tsi, p);
+}
+
 template <int LOGN>
 static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
@@ -447,6 +529,14 @@ int cp_homomul_setup(cupoly_engine *e) {
     case 11: return homomul_setup_t<11>();
     case 12: return homomul_setup_t<12>();
     case 13: return homomul_setup_t<13>();
+    case 14: {
+      const int lds = (int)(lds_row<13>() * sizeof(cp_gi));
+      CP_HIP(hipFuncSetAttribute((const void *)k_dgt_fwd_tm16,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      CP_HIP(hipFuncSetAttribute((const void *)k_tensor_idgt16,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      return CUPOLY_OK;
+    }
     default: return CUPOLY_OK;  // other sizes use the generic transforms
   }
 }
@@ -567,7 +657,11 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   if (rc) return rc;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
-  const bool fused_sizes = e->logng >= 10 && e->logng <= 13;
+  // n = 2^15 uses the one-workgroup-per-polynomial kernels once the batch has
+  // at least one forward residue polynomial per CU; smaller batches keep the
+  // split transforms, which put more workgroups on each polynomial
+  const bool big16 = e->logng == 14 && batch * 4 * nres >= e->ncu;
+  const bool fused_sizes = (e->logng >= 10 && e->logng <= 13) || big16;
   // per-residue threads while the per-slot forms would leave the GPU mostly idle
   const bool wide = (long)batch * 3 * ng < c->wide_slots;
   if (wide)
@@ -577,7 +671,16 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1,
                        ct2, ng, e->d_hps, !fused_sizes);
   CP_LAUNCH_CHECK();
-  if (fused_sizes) {
+  if (big16) {
+    const size_t lds = lds_row<13>() * sizeof(cp_gi);
+    hipLaunchKernelGGL(k_dgt_fwd_tm16, dim3(batch * 4 * nres), dim3(1024), lds, c->stream, X,
+                       batch * 4 * nres, nres, k, ct1, ct2, e->d_primes, e->d_tw_fwd,
+                       e->d_twist_fwd);
+    CP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tensor_idgt16, dim3(batch * 3 * nres), dim3(1024), lds, c->stream, Y, X,
+                       batch * 3 * nres, nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+    CP_LAUNCH_CHECK();
+  } else if (fused_sizes) {
     if ((rc = fwd_tm(c, X, batch * 4 * nres, ct1, ct2))) return rc;
     if ((rc = tensor_idgt(c, Y, X, batch * 3 * nres))) return rc;
   } else {

@@ -300,12 +300,15 @@ def test_large_batch_properties(gpu):
 
 @pytest.mark.parametrize("k,n,batch", [(1, 64, 3), (2, 2048, 3), (3, 4096, 2), (5, 8192, 1),
                                          (4, 16384, 2), (2, 32768, 1), (2, 2048, 48),
-                                         (8, 32768, 3)])
+                                         (8, 32768, 3), (8, 32768, 4), (2, 32768, 13)])
 def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
-    """n = 2^14, k = 4, kl = 5, t = 65537 (and smaller rings): tensor product + HPS
+    """n = 2^14, k = 4, kl = 5, t = 65537 (and other rings): tensor product + HPS
     rescale of ciphertext pairs, bit-exact against the oracle composition.
     Batches under 2^16 output slots take the per-residue HPS kernels, the
-    others (48 and 3 pairs among them) the per-slot ones."""
+    others (48 and 3 pairs among them) the per-slot ones.  At n = 2^15, batches
+    with at least one forward residue polynomial per CU (8 x 4 x 17 and
+    13 x 4 x 5 on 256 CUs) take the one-workgroup-per-polynomial transforms,
+    smaller ones the split transforms."""
     e, ctx = engine(k, k + 1, n, 65537)
     P = O.Params(k, k + 1, n, 65537)
     ct1 = rand_res(P, P.q, (batch, 2), 90 + k)
