@@ -103,13 +103,11 @@ __global__ void __launch_bounds__(256)
     acc_i = fma((double)li, two_m64, acc_i);
   }
   const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
-  // B -> Q sums, exact (every term is reduced) and reduced once per output
-  cp_acc sr[K], si[K];
+  // B -> Q sums, one modular word per output residue: K 160-bit accumulators
+  // would stay live across the whole B-residue loop (272 VGPRs at K = 8)
+  uint64_t sr[K], si[K];
 #pragma unroll
-  for (int j = 0; j < K; j++) {
-    cp_acc_init(sr[j]);
-    cp_acc_init(si[j]);
-  }
+  for (int j = 0; j < K; j++) sr[j] = si[j] = 0;
   double vr = 0.0, vi = 0.0;
   for (int i = 0; i < kl; i++) {
     const uint64_t b = H->b[i].p;
@@ -131,21 +129,19 @@ __global__ void __launch_bounds__(256)
     vi = vi + (double)ai / bd;
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      cp_acc_mac(sr[j], ar, H->Bbiqi[j][i][0]);
-      cp_acc_mac(si[j], ai, H->Bbiqi[j][i][0]);
+      const uint64_t q = H->q[j].p;
+      sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
+      si[j] = addmod_ref(si[j], shoup2(ai, H->Bbiqi[j][i], q), q);
     }
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   cp_gi *o = out.at(pair) + (size_t)m * K * ng;
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const uint64_t q = H->q[j].p, bq = H->Bqi[j][0];
-    const uint64_t nbq = bq ? q - bq : 0;  // -B mod q_j
-    cp_acc_mac(sr[j], v_re, nbq);
-    cp_acc_mac(si[j], v_im, nbq);
+    const uint64_t q = H->q[j].p;
     cp_gi r;
-    r.re = cp_acc_reduce(sr[j], q);
-    r.imag = cp_acc_reduce(si[j], q);
+    r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
+    r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
     o[(size_t)j * ng + c] = r;
   }
 }
