@@ -654,9 +654,11 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
   // n = 2^15 uses the one-workgroup-per-polynomial kernels once the batch has
-  // at least one forward residue polynomial per CU; smaller batches keep the
-  // split transforms, which put more workgroups on each polynomial
-  const bool big16 = e->logng == 14 && batch * 4 * nres >= e->ncu;
+  // forward residue polynomials for three quarters of the CUs; smaller
+  // batches keep the split transforms, which put more workgroups on each
+  // polynomial (measured at k = 8: 2 pairs split 0.24 ms against 0.26 ms,
+  // 3 pairs 0.32 against 0.28 ms)
+  const bool big16 = e->logng == 14 && 4 * batch * 4 * nres >= 3 * e->ncu;
   const bool fused_sizes = (e->logng >= 10 && e->logng <= 13) || big16;
   // per-residue threads while the per-slot forms would leave the GPU mostly idle
   const bool wide = (long)batch * 3 * ng < c->wide_slots;

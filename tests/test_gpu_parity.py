@@ -306,9 +306,9 @@ def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
     rescale of ciphertext pairs, bit-exact against the oracle composition.
     Batches under 2^16 output slots take the per-residue HPS kernels, the
     others (48 and 3 pairs among them) the per-slot ones.  At n = 2^15, batches
-    with at least one forward residue polynomial per CU (8 x 4 x 17 and
-    13 x 4 x 5 on 256 CUs) take the one-workgroup-per-polynomial transforms,
-    smaller ones the split transforms."""
+    with forward residue polynomials for three quarters of the CUs (3 x 4 x 17,
+    4 x 4 x 17 and 13 x 4 x 5 on 256 CUs) take the one-workgroup-per-polynomial
+    transforms, smaller ones (1 x 4 x 5) the split transforms."""
     e, ctx = engine(k, k + 1, n, 65537)
     P = O.Params(k, k + 1, n, 65537)
     ct1 = rand_res(P, P.q, (batch, 2), 90 + k)
