@@ -300,11 +300,15 @@ extern "C" int cupoly_uneven_create(cupoly_ctx *c, cupoly_engine *const *engines
       u->units_h.push_back(un);
     }
   }
-  // tiles: consecutive units of one class; reorder units so each tile's units are contiguous
+  // tiles: consecutive units of one class; reorder units so each tile's units
+  // are contiguous.  Within a class the units go residue-major, so a tile's
+  // rows and consecutive tiles share one residue's tables in L2.
   std::vector<cp_unit> ordered;
   for (int cl = 0; cl < UNEVEN_NCLASS; cl++) {
     const int per_tile = cl == 4 ? 1 : UNEVEN_SLOTS >> (cl + 10);
-    const auto &ids = per_cls[cl];
+    auto &ids = per_cls[cl];
+    std::stable_sort(ids.begin(), ids.end(),
+                     [&](int x, int y) { return u->units_h[x].rid < u->units_h[y].rid; });
     for (size_t s = 0; s < ids.size(); s += per_tile) {
       cp_tile tl;
       tl.cls = cl;
