@@ -601,7 +601,10 @@ extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupo
   if (cand.empty()) cand.push_back({0, 0});
   hipEvent_t e0, e1;
   CP_HIP(hipEventCreate(&e0));
-  CP_HIP(hipEventCreate(&e1));
+  if (hipError_t err = hipEventCreate(&e1); err != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return cp_check_hip(err, "hipEventCreate");
+  }
   auto hip = [](hipError_t err, const char *what) {
     return err == hipSuccess ? CUPOLY_OK : cp_check_hip(err, what);
   };

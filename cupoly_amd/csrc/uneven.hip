@@ -437,7 +437,10 @@ extern "C" int cupoly_uneven_autotune(cupoly_uneven *u, int reps, int *best_grid
   const int grids[] = {cus / 2, cus, 2 * cus, 4 * cus};
   hipEvent_t e0, e1;
   CP_HIP(hipEventCreate(&e0));
-  CP_HIP(hipEventCreate(&e1));
+  if (hipError_t err = hipEventCreate(&e1); err != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return cp_check_hip(err, "hipEventCreate");
+  }
   float best = 1e30f;
   int bg = u->grid, bo = u->order, rc = CUPOLY_OK;
   for (int order = 0; order < 2 && !rc; order++)
