@@ -58,6 +58,29 @@ inline void check(int rc) {
 inline void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+// A temporary device buffer, freed on every exit path.  hipFree waits for the
+// device work still using it, so an exception between its uses and the
+// caller's stream synchronisation leaks nothing and frees nothing early.
+struct DeviceTemp {
+  void *p = nullptr;
+  DeviceTemp(size_t bytes, const char *what) { hip_check(hipMalloc(&p, bytes), what); }
+  ~DeviceTemp() {
+    if (p) (void)hipFree(p);
+  }
+  DeviceTemp(const DeviceTemp &) = delete;
+  DeviceTemp &operator=(const DeviceTemp &) = delete;
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+// Waits for a stream when it goes out of scope, so that host buffers declared
+// before it outlive every asynchronous copy into or out of them, whichever
+// way the function is left.
+struct SyncOnExit {
+  hipStream_t s;
+  ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+};
 }  // namespace cupoly_detail
 
 // CUDAEngine: the reference's process-wide engine (cudaengine.h:65-407),
@@ -381,19 +404,19 @@ inline void poly_simple_scaling_tDivQ(Context *ctx, poly_t *b, poly_t *a) {
 // residue), as poly_export_residues lays them out (polynomial.cu:1321-1330).
 inline std::vector<uint64_t> poly_get_residues(Context *ctx, poly_t *a) {
   const int nres = CUDAEngine::get_n_residues(a->base), N = CUDAEngine::N;
-  GaussianInteger *tmp = nullptr;
   const size_t bytes = poly_get_residues_size(a->base);
-  cupoly_detail::hip_check(hipMalloc((void **)&tmp, bytes), "poly_get_residues");
+  cupoly_detail::DeviceTemp buf(bytes, "poly_get_residues");
+  GaussianInteger *tmp = buf.as<GaussianInteger>();
   cupoly_detail::hip_check(
       hipMemcpyAsync(tmp, a->d_coefs, bytes, hipMemcpyDeviceToDevice, ctx->get_stream()),
       "poly_get_residues");
   cupoly_detail::check(cupoly_dgt(ctx->ctx, tmp, a->base, INVERSE));
   std::vector<GaussianInteger> h((size_t)nres * N);
+  cupoly_detail::SyncOnExit guard{ctx->get_stream()};
   cupoly_detail::hip_check(
       hipMemcpyAsync(h.data(), tmp, bytes, hipMemcpyDeviceToHost, ctx->get_stream()),
       "poly_get_residues");
   ctx->sync();
-  (void)hipFree(tmp);
   std::vector<uint64_t> out((size_t)nres * 2 * N);
   for (int r = 0; r < nres; r++)
     for (int c = 0; c < N; c++) {
@@ -446,12 +469,19 @@ inline poly_t *poly_import_residues(Context *ctx, const std::vector<uint64_t> &v
       h[(size_t)rid * N + cid - N].imag = v[i];
   }
   poly_t *a = new poly_t;
-  poly_init(ctx, a, base);
-  cupoly_detail::hip_check(hipMemcpyAsync(a->d_coefs, h.data(), poly_get_residues_size(base),
-                                          hipMemcpyHostToDevice, ctx->get_stream()),
-                           "poly_import_residues");
-  cupoly_detail::check(cupoly_dgt(ctx->ctx, a->d_coefs, base, FORWARD));
-  ctx->sync();
+  try {
+    cupoly_detail::SyncOnExit guard{ctx->get_stream()};
+    poly_init(ctx, a, base);
+    cupoly_detail::hip_check(hipMemcpyAsync(a->d_coefs, h.data(), poly_get_residues_size(base),
+                                            hipMemcpyHostToDevice, ctx->get_stream()),
+                             "poly_import_residues");
+    cupoly_detail::check(cupoly_dgt(ctx->ctx, a->d_coefs, base, FORWARD));
+    ctx->sync();
+  } catch (...) {
+    poly_free(ctx, a);
+    delete a;
+    throw;
+  }
   a->status = RNSSTATE;
   return a;
 }
@@ -472,39 +502,42 @@ inline poly_t *poly_import_residues(Context *ctx, const std::string &s, int base
 inline poly_t *poly_import_integers(Context *ctx, const std::vector<uint64_t> &coefs, int limbs,
                                     int base = QBase) {
   poly_t *p = new poly_t;
-  poly_init(ctx, p, base);
-  const int ncoefs = (int)(coefs.size() / limbs);
-  uint64_t *d = nullptr;
-  cupoly_detail::hip_check(hipMalloc((void **)&d, coefs.size() * 8 + 8), "poly_import_integers");
-  cupoly_detail::hip_check(hipMemcpyAsync(d, coefs.data(), coefs.size() * 8,
-                                          hipMemcpyHostToDevice, ctx->get_stream()),
-                           "poly_import_integers");
-  cupoly_detail::check(cupoly_crt(ctx->ctx, p->d_coefs, d, ncoefs, limbs, base));
-  cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, base, FORWARD));
-  ctx->sync();
-  (void)hipFree(d);
+  try {
+    poly_init(ctx, p, base);
+    const int ncoefs = (int)(coefs.size() / limbs);
+    cupoly_detail::DeviceTemp buf(coefs.size() * 8 + 8, "poly_import_integers");
+    uint64_t *d = buf.as<uint64_t>();
+    cupoly_detail::hip_check(hipMemcpyAsync(d, coefs.data(), coefs.size() * 8,
+                                            hipMemcpyHostToDevice, ctx->get_stream()),
+                             "poly_import_integers");
+    cupoly_detail::check(cupoly_crt(ctx->ctx, p->d_coefs, d, ncoefs, limbs, base));
+    cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, base, FORWARD));
+    ctx->sync();
+  } catch (...) {
+    poly_free(ctx, p);
+    delete p;
+    throw;
+  }
   p->status = RNSSTATE;
   return p;
 }
 inline std::vector<uint64_t> poly_get_integers(Context *ctx, poly_t *a) {
   const int k = CUDAEngine::get_n_residues(QBase), N = CUDAEngine::N;
-  GaussianInteger *tmp = nullptr;
-  uint64_t *d = nullptr;
   const size_t bytes = poly_get_residues_size(QBase), obytes = (size_t)2 * N * k * 8;
-  cupoly_detail::hip_check(hipMalloc((void **)&tmp, bytes), "poly_get_integers");
-  cupoly_detail::hip_check(hipMalloc((void **)&d, obytes), "poly_get_integers");
+  cupoly_detail::DeviceTemp tbuf(bytes, "poly_get_integers"), dbuf(obytes, "poly_get_integers");
+  GaussianInteger *tmp = tbuf.as<GaussianInteger>();
+  uint64_t *d = dbuf.as<uint64_t>();
   cupoly_detail::hip_check(
       hipMemcpyAsync(tmp, a->d_coefs, bytes, hipMemcpyDeviceToDevice, ctx->get_stream()),
       "poly_get_integers");
   cupoly_detail::check(cupoly_dgt(ctx->ctx, tmp, QBase, INVERSE));
   cupoly_detail::check(cupoly_icrt(ctx->ctx, d, tmp));
   std::vector<uint64_t> out((size_t)2 * N * k);
+  cupoly_detail::SyncOnExit guard{ctx->get_stream()};
   cupoly_detail::hip_check(
       hipMemcpyAsync(out.data(), d, obytes, hipMemcpyDeviceToHost, ctx->get_stream()),
       "poly_get_integers");
   ctx->sync();
-  (void)hipFree(tmp);
-  (void)hipFree(d);
   return out;
 }
 
@@ -523,9 +556,8 @@ inline void poly_select_residue(Context *ctx, poly_t *p, int id) {
   if (!p->init || id < 0 || id >= nres)
     throw std::runtime_error("poly_select_residue: bad argument");
   if (nres == 1) return;
-  uint64_t *d = nullptr;
-  cupoly_detail::hip_check(hipMalloc((void **)&d, (size_t)2 * N * sizeof(uint64_t)),
-                           "poly_select_residue");
+  cupoly_detail::DeviceTemp buf((size_t)2 * N * sizeof(uint64_t), "poly_select_residue");
+  uint64_t *d = buf.as<uint64_t>();
   cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, p->base, INVERSE));
   // residue id as 2N one-limb integers: coefficient j from re[j], j + N from imag[j]
   const GaussianInteger *r = p->d_coefs + (size_t)id * N;
@@ -540,7 +572,6 @@ inline void poly_select_residue(Context *ctx, poly_t *p, int id) {
   cupoly_detail::check(cupoly_crt(ctx->ctx, p->d_coefs, d, 2 * N, 1, p->base));
   cupoly_detail::check(cupoly_dgt(ctx->ctx, p->d_coefs, p->base, FORWARD));
   ctx->sync();
-  (void)hipFree(d);
 }
 
 // poly_discard_qbase (polynomial.cu:1645-1660): a QB polynomial keeps only its
