@@ -179,7 +179,7 @@ int cupoly_dot(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *const *a, const c
  * out[i] = a[i] * b[i] mod (X^n + 1, q_r) for `batch` products of the engine's
  * base-Q shape, coefficient domain in and out, [batch][k][n/2] GIs; out may be
  * a or b.  One fused launch: forward DGT of a and b, pointwise product,
- * inverse DGT. */
+ * inverse DGT.  batch * k must stay below 2^31 (CUPOLY_ERR_ARG otherwise). */
 int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, const cupoly_gi *b,
                          int batch);
 /* Launch choices of cupoly_polymul_batch on this context (no reference
@@ -207,7 +207,8 @@ int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi 
  * pair i: ct1[i] = (c0, c1), ct2[i] = (d0, d1), [batch][2][k][n/2] GIs, base Q,
  * coefficient domain; out[i] = [3][k][n/2]: round(t/Q (c0 d0, c0 d1 + c1 d0,
  * c1 d1)) mod Q, coefficient domain.  Uses a context-owned workspace of
- * batch * 7 * (k+kl) * n/2 GIs (allocated on first use, grown as needed). */
+ * batch * 7 * (k+kl) * n/2 GIs (allocated on first use, grown as needed).
+ * batch <= 16383 (CUPOLY_ERR_ARG otherwise). */
 int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, const cupoly_gi *ct2,
                          int batch);
 /* Same computation for gathered operands (a serving batch of requests whose
@@ -229,7 +230,8 @@ int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
  * poly_mul_add, sampler.cu:576-635, polynomial.cu:916-948); a, out
  * [batch][k][n/2].  For n <= 16384 two launches: s_i and e_i drawn once per
  * instance into the context's workspace (4 bytes per slot), then one fused
- * transform + multiply-add launch per residue. */
+ * transform + multiply-add launch per residue.  batch * k must stay below
+ * 2^31. */
 int cupoly_sample_muladd_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, int batch,
                                uint64_t seed, uint64_t stream0);
 
