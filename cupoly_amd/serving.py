@@ -86,6 +86,12 @@ class Server:
     def __init__(self, shapes, hbm_budget, max_batch=None, max_inflight=2):
         self.shapes = shapes
         self.hbm_budget = hbm_budget
+        # a request larger than the whole budget could never be admitted, and
+        # the loop would wait for it forever
+        for sh in shapes:
+            if sh.footprint > hbm_budget:
+                raise ValueError("HBM budget of %d bytes is below the footprint of one %r request "
+                                 "(%d bytes)" % (hbm_budget, sh, sh.footprint))
         max_batch = max_batch or {sh: 64 for sh in shapes}
         # a batch never holds more requests than the budget admits at once
         self.max_batch = {sh: max(1, min(max_batch[sh], hbm_budget // sh.footprint))
@@ -96,6 +102,10 @@ class Server:
                 cp.hom_mul_reserve(ctx, self.max_batch[sh])
 
     def run(self, requests):
+        if not requests:
+            return {"requests": 0, "batches": 0, "req_per_s": 0.0, "p50_ms": None,
+                    "p99_ms": None, "mean_ms": None, "max_ms": None,
+                    "peak_admitted_bytes": 0, "per_shape": {}}
         import torch
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queues = {sh: collections.deque() for sh in self.shapes}
