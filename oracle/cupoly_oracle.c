@@ -21,9 +21,22 @@
  *                                from the 1e-9 snap window; i^2 mod t KAT.
  *   - or_complex_scaling       : exact round(t*x/Q) mod b_i within +-1, as the
  *                                reference's own test allows.
+ *   - or_simple_scaling /
+ *     or_scale_residue         : exact round(t*x/Q) mod t (Python ints), and
+ *                                scaled messages recovered for t = 65537.
+ *   - or_pointwise             : Gaussian-integer arithmetic in Python ints.
+ *   - or_dgt_poly              : the reference's hierarchical kernels restated
+ *                                in Python; full bit reversal of the placement.
+ *   - or_negacyclic_schoolbook : Python big-integer negacyclic products.
  *   - or_fma_rd / or_rcp_rd    : exact rational rounding (Python fractions).
  *   - or_philox4x32_10         : Random123 known-answer vectors.
- *   - samplers                 : chi-square against the ideal distributions.
+ *   - samplers                 : chi-square against the ideal distributions;
+ *                                the fixed-weight sampler also by its exact
+ *                                weight, uniform positions and signs, and a
+ *                                brute-force smallest-key selection.
+ * The word decomposition (xi, rho, dot) and the CRT are restated in
+ * oracle.py and pinned there by sum_i rho(a)_i xi(b)_i = a*b and by the CRT
+ * identity against Python integers.
  *
  * Layout conventions (same as the reference, src/arithmetic/polynomial.cu:190-221):
  *   a polynomial of ring degree n = 2*Ng over nres residues is stored as
