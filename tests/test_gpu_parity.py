@@ -320,6 +320,31 @@ def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
         assert np.array_equal(h[i], O.hom_mul(P, ct1[i], ct2[i])), i
 
 
+def test_hom_mul_full_batch_properties(gpu):
+    """n = 2^14, k = 4, 128 pairs (the benchmarked batch): the tensor product is
+    symmetric, so swapping the ciphertexts leaves every pair's result unchanged;
+    three pairs run alone (a batch of one takes the per-residue HPS kernels,
+    the full batch the per-slot ones) give the same residues as in the batch;
+    one pair equals the oracle composition."""
+    import torch
+    k, n, batch = 4, 16384, 128
+    e, ctx = engine(k, k + 1, n, 65537)
+    P = O.Params(k, k + 1, n, 65537)
+    ct1 = rand_res(P, P.q, (batch, 2), 501)
+    ct2 = rand_res(P, P.q, (batch, 2), 502)
+    d1, d2 = cp.to_device(ct1), cp.to_device(ct2)
+    out = cp.empty((batch, 3, k, P.Ng, 2))
+    swapped = cp.empty(out.shape)
+    cp.hom_mul_batch(ctx, out, d1, d2)
+    cp.hom_mul_batch(ctx, swapped, d2, d1)
+    assert torch.equal(out, swapped)
+    one = cp.empty((1, 3, k, P.Ng, 2))
+    for i in (0, 77, 127):
+        cp.hom_mul_batch(ctx, one, d1[i:i + 1], d2[i:i + 1])
+        assert torch.equal(one[0], out[i]), i
+    assert np.array_equal(cp.to_host(out[5]), O.hom_mul(P, ct1[5], ct2[5]))
+
+
 @pytest.mark.parametrize("k,n,batch", [(2, 2048, 5), (4, 16384, 3), (3, 32768, 2)])
 def test_sample_muladd_matches_oracle(gpu, k, n, batch):
     """n = 2^14, k = 4 (and other rings): a * s + e with s ternary and e discrete
