@@ -345,6 +345,41 @@ def test_hom_mul_full_batch_properties(gpu):
     assert np.array_equal(cp.to_host(out[5]), O.hom_mul(P, ct1[5], ct2[5]))
 
 
+def test_sample_muladd_full_batch_properties(gpu):
+    """n = 2^14, k = 4, batch 1024 (the benchmarked batch): item i of the batch
+    uses the counters stream0 + 2i (s) and stream0 + 2i + 1 (e), so three items
+    recomputed alone with those counters give the same residues; one item
+    equals the oracle.  With a = 0 every item is the spectrum of its Gaussian
+    e alone: back in the coefficient domain all 1024 * n coefficients are one
+    small integer in every residue, within the sampler's truncation bound."""
+    import torch
+    k, n, batch, seed, s0 = 4, 16384, 1024, 777, 40
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 611)
+    da = cp.to_device(a)
+    out = cp.empty(a.shape)
+    cp.sample_muladd_batch(ctx, out, da, seed, s0)
+    one = cp.empty((1,) + a.shape[1:])
+    for i in (0, 513, 1023):
+        cp.sample_muladd_batch(ctx, one, da[i:i + 1], seed, s0 + 2 * i)
+        assert torch.equal(one[0], out[i]), i
+    assert np.array_equal(cp.to_host(out[7]), O.sample_muladd(P, a[7:8], seed, s0 + 14)[0])
+    del da, out
+    da = cp.empty(a.shape)
+    noise = cp.empty(a.shape)
+    cp.sample_muladd_batch(ctx, noise, da, seed, s0)
+    for i in range(batch):
+        cp.execute_dgt(ctx, noise[i], cp.QBase, cp.INVERSE)
+    h = cp.to_host(noise)
+    q = np.array(P.q, dtype=np.uint64).reshape(1, k, 1, 1)
+    centered = np.where(h > q // 2, -((q - h).astype(np.int64)), h.astype(np.int64))
+    _, bound, _ = O.cdt()
+    assert np.abs(centered).max() <= bound
+    assert (centered == centered[:, :1]).all()  # the same integer in every residue
+    assert np.count_nonzero(centered) > batch * n // 2  # not all zero
+
+
 @pytest.mark.parametrize("k,n,batch", [(2, 2048, 5), (4, 16384, 3), (3, 32768, 2)])
 def test_sample_muladd_matches_oracle(gpu, k, n, batch):
     """n = 2^14, k = 4 (and other rings): a * s + e with s ternary and e discrete
