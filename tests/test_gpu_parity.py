@@ -298,6 +298,27 @@ def test_large_batch_properties(gpu):
         assert np.array_equal(h[i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
 
 
+def test_polymul_full_batch_properties(gpu):
+    """n = 2^14, k = 4, 1024 products (the benchmarked launch): every fused
+    product equals the one-launch-per-op form, a * b = b * a for every
+    product, and two products equal the oracle."""
+    import torch
+    k, n, batch = 4, 16384, 1024
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 81)
+    b = rand_res(P, P.q, (batch,), 82)
+    da, db = cp.to_device(a), cp.to_device(b)
+    o1, o2 = cp.empty(a.shape), cp.empty(a.shape)
+    cp.polymul_batch(ctx, o1, da, db)
+    cp.polymul_batch(ctx, o2, db, da)
+    assert torch.equal(o1, o2)
+    cp.polymul_plain(ctx, o2, da.clone(), db.clone())
+    assert torch.equal(o1, o2)
+    for i in (3, 1000):
+        assert np.array_equal(cp.to_host(o1[i]), O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0]), i
+
+
 @pytest.mark.parametrize("k,n,batch", [(1, 64, 3), (2, 2048, 3), (3, 4096, 2), (5, 8192, 1),
                                          (4, 16384, 2), (2, 32768, 1), (2, 2048, 48),
                                          (8, 32768, 3), (8, 32768, 4), (2, 32768, 13)])
