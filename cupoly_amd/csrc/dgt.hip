@@ -610,6 +610,7 @@ extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupo
   };
   float best = -1.0f;
   std::pair<int, int> pick = cand[0];
+  const std::pair<int, int> before{c->tune_rows, c->tune_persistent};
   int rc = CUPOLY_OK;
   for (const auto &cd : cand) {
     c->tune_rows = cd.first;
@@ -629,9 +630,13 @@ extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupo
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) {  // a failed search leaves the caller's choice in place
+    c->tune_rows = before.first;
+    c->tune_persistent = before.second;
+    return rc;
+  }
   c->tune_rows = pick.first;
   c->tune_persistent = pick.second;
-  if (rc) return rc;
   if (rows) *rows = pick.first;
   if (persistent) *persistent = pick.second;
   return cupoly_polymul_batch(c, out, a, b, batch);

@@ -441,19 +441,24 @@ extern "C" int cupoly_uneven_autotune(cupoly_uneven *u, int reps, int *best_grid
     (void)hipEventDestroy(e0);
     return cp_check_hip(err, "hipEventCreate");
   }
+  auto hip = [](hipError_t err, const char *what) {
+    return err == hipSuccess ? CUPOLY_OK : cp_check_hip(err, what);
+  };
   float best = 1e30f;
-  int bg = u->grid, bo = u->order, rc = CUPOLY_OK;
+  const int grid0 = u->grid, order0 = u->order;
+  int bg = grid0, bo = order0, rc = CUPOLY_OK;
   for (int order = 0; order < 2 && !rc; order++)
     for (int gi = 0; gi < 4 && !rc; gi++) {
       const int g = std::max(1, std::min(grids[gi], std::max(1, u->ntiles)));
       if ((rc = cupoly_uneven_set_tuning(u, g, order))) break;
       if ((rc = launch_uneven(u, u->ctx->stream))) break;  // warm-up
-      (void)hipEventRecord(e0, u->ctx->stream);
+      if ((rc = hip(hipEventRecord(e0, u->ctx->stream), "hipEventRecord"))) break;
       for (int r = 0; r < reps && !rc; r++) rc = launch_uneven(u, u->ctx->stream);
-      (void)hipEventRecord(e1, u->ctx->stream);
-      (void)hipEventSynchronize(e1);
+      if (rc || (rc = hip(hipEventRecord(e1, u->ctx->stream), "hipEventRecord")) ||
+          (rc = hip(hipEventSynchronize(e1), "hipEventSynchronize")))
+        break;
       float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
+      if ((rc = hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))) break;
       ms /= reps;
       if (ms < best) {
         best = ms;
@@ -463,7 +468,11 @@ extern "C" int cupoly_uneven_autotune(cupoly_uneven *u, int reps, int *best_grid
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc) return rc;
+  if (rc) {  // a failed search leaves the caller's choice in place
+    const std::string msg = cupoly_last_error();
+    (void)cupoly_uneven_set_tuning(u, grid0, order0);
+    return cp_fail(rc, msg);
+  }
   if ((rc = cupoly_uneven_set_tuning(u, bg, bo))) return rc;
   if (best_grid) *best_grid = bg;
   if (best_order) *best_order = bo;
