@@ -712,6 +712,8 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   if (c && batch == 0) return CUPOLY_OK;  // empty batch: tables may be null
   if (!c || !out || !ct1 || !ct2 || batch < 0)
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
+  // the limit hom_mul_run enforces, checked before the tables grow
+  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   for (int i = 0; i < batch; i++)
     if (!out[i] || !ct1[i] || !ct2[i]) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: null entry");
   // pointer tables to the device on the context's stream, from a pinned

@@ -1,6 +1,8 @@
 """Size-independent properties of the batched product at full ring sizes,
 extreme residues and empty batches (the oracle is too slow for whole batches
 at n = 2^14 and 2^15; these checks need no oracle)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -155,4 +157,11 @@ def test_oversized_batch_is_rejected_before_any_launch(gpu):
     assert "2^31" in cp.lib().cupoly_last_error().decode()
     assert cp.lib().cupoly_sample_muladd_batch(ctx.handle, p, p, 1 << 30, 1, 0) < 0
     assert "2^31" in cp.lib().cupoly_last_error().decode()
+    # hom-mul: at most 16383 pairs, in the contiguous and the gathered form
+    big = 16384
+    assert cp.lib().cupoly_hom_mul_batch(ctx.handle, p, p, p, big) < 0
+    assert "16383" in cp.lib().cupoly_last_error().decode()
+    tbl = (ctypes.c_void_p * big)(*([p.value] * big))
+    assert cp.lib().cupoly_hom_mul_gather(ctx.handle, tbl, tbl, tbl, big) < 0
+    assert "16383" in cp.lib().cupoly_last_error().decode()
     ctx.sync()
