@@ -121,10 +121,20 @@ def _check(rc):
 
 
 def _ptr(t):
+    """Device address of a buffer the kernels read or write as 64-bit words.
+    Every pointer handed to the library goes through here: the element type
+    must be 8 bytes wide (int64 or uint64; any other type of the same shape
+    would be over-run by the kernels), the storage contiguous and on the GPU.
+    Checked in that order, each with its own ValueError."""
     if t is None:
         return None
-    if not t.is_cuda or not t.is_contiguous():
-        raise ValueError("cupoly_amd: expected a contiguous device tensor")
+    import torch
+    if t.dtype not in (torch.int64, torch.uint64):
+        raise ValueError("cupoly_amd: expected an int64 or uint64 tensor, got %s" % (t.dtype,))
+    if not t.is_contiguous():
+        raise ValueError("cupoly_amd: expected a contiguous tensor")
+    if not t.is_cuda:
+        raise ValueError("cupoly_amd: expected a device (GPU) tensor, got one on %s" % (t.device,))
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -264,6 +274,9 @@ def execute_polynomial_op_by_int(ctx, b, a, x, op, base=QBase):
 
 
 def execute_polynomial_double_op_by_int(ctx, b1, a1, x1, b2, a2, x2, op, base=QBase):
+    e = ctx.engine
+    for t in (b1, a1, b2, a2):
+        _need(t, e.get_n_residues(base), e.N)
     _check(lib().cupoly_double_op_by_int(ctx.handle, _ptr(b1), _ptr(a1), x1, _ptr(b2),
                                          _ptr(a2), x2, op, base))
 
@@ -365,7 +378,7 @@ def hom_mul_gather(ctx, outs, ct1s, ct2s):
         if tuple(t.shape) != (m, e.k, e.N, 2):
             raise ValueError("cupoly_amd: hom_mul_gather operand shape %s" % (tuple(t.shape),))
     arr = ctypes.c_void_p * max(b, 1)
-    po = arr(*[t.data_ptr() for t in outs])
+    po = arr(*[_ptr(t).value for t in outs])
     p1 = arr(*[_ptr(t).value for t in ct1s])
     p2 = arr(*[_ptr(t).value for t in ct2s])
     _check(lib().cupoly_hom_mul_gather(ctx.handle, po, p1, p2, b))
@@ -511,8 +524,8 @@ class UnevenPlan:
         h = ctypes.c_void_p()
         _check(lib().cupoly_uneven_create(
             ctx.handle, eng, len(engines), arri(*[it[0] for it in items]),
-            arri(*[it[1] for it in items]), arrp(*[it[2].data_ptr() for it in items]),
-            arrp(*[it[3].data_ptr() for it in items]), arrp(*[it[4].data_ptr() for it in items]),
+            arri(*[it[1] for it in items]), arrp(*[_ptr(it[2]).value for it in items]),
+            arrp(*[_ptr(it[3]).value for it in items]), arrp(*[_ptr(it[4]).value for it in items]),
             cnt, ctypes.byref(h)))
         self.handle = h
 
