@@ -70,6 +70,8 @@ SIGNATURES = {
     "cupoly_hom_mul_batch": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_gather": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_reserve": ([_vp, _int], _int),
+    "cupoly_hom_mul_workspace_bytes": ([_vp, _int, _vp], _int),
+    "cupoly_hom_mul_set_tuning": ([_vp, ctypes.c_long, _int], _int),
     "cupoly_polymul_set_tuning": ([_vp, _int, _int], _int),
     "cupoly_polymul_autotune": ([_vp, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
@@ -413,10 +415,21 @@ def hom_mul_reserve(ctx, batch):
     _check(lib().cupoly_hom_mul_reserve(ctx.handle, int(batch)))
 
 
+def hom_mul_set_tuning(ctx, wide_slots=0, scale_chain=0):
+    """Launch choices of hom_mul_batch / hom_mul_gather on ctx: the output-slot
+    count below which the HPS steps run one thread per (slot, residue) (0 =
+    built-in), and scale_chain = 1 for the reference's addmod chain in every
+    t/Q residue instead of the exact sums (same words)."""
+    _check(lib().cupoly_hom_mul_set_tuning(ctx.handle, int(wide_slots), int(scale_chain)))
+
+
 def hom_mul_workspace_bytes(engine, batch):
-    """Device bytes hom_mul_reserve(ctx, batch) holds for engine's shape:
-    7 (k + kl) residue polynomials of N Gaussian slots per pair."""
-    return batch * 7 * (engine.k + engine.kl) * engine.N * 16
+    """Device bytes hom_mul_reserve(ctx, batch) holds for engine's shape
+    (the library's own figure: residue polynomials of N Gaussian slots for the
+    extended inputs and the three products of every pair)."""
+    out = ctypes.c_uint64()
+    _check(lib().cupoly_hom_mul_workspace_bytes(engine.handle, int(batch), ctypes.byref(out)))
+    return out.value
 
 
 def sample_muladd_batch(ctx, out, a, seed, stream0):

@@ -12,14 +12,14 @@
 // Pipeline, every stage batched over all ciphertext pairs (one launch each):
 //   1. k_ext_q_to_qb:   each of the 4 inputs -> its B residues by the HPS
 //                       extension, one thread per Gaussian slot (the Q rows
-//                       are read from the input by the forward transform,
-//                       or copied for n = 2^15);
-//   2. forward DGT of the 4*(k+kl) residue polynomials per pair;
-//   3.+4. k_tensor_idgt: slot-wise c0 d0, c0 d1 + c1 d0, c1 d1 formed as the
-//                       inverse DGT of each of the 3*(k+kl) results loads its
-//                       operands (n <= 2^14; above, k_tensor then the
-//                       two-kernel inverse DGT);
-//   5. k_scale_to_q:    complex scaling t/Q (QB -> B) and B -> Q fused in
+//                       are read from the input by the transforms, or copied
+//                       for the generic sizes);
+//   2. n = 2^11 .. 2^14, k_hom_fti: one workgroup per (pair, residue) runs
+//                       the 4 forward transforms, the slot-wise tensor
+//                       product and the 3 inverse transforms;
+//      n = 2^15: forward transforms (k_dgt_fwd_tm16), then the tensor
+//                       product fused into the inverse (k_tensor_idgt16);
+//   3. k_scale_to_q:    complex scaling t/Q (QB -> B) and B -> Q fused in
 //                       registers, one thread per slot, writing base Q.
 // Scratch lives in a grow-only workspace owned by the context.
 #include "dgt_dev.hpp"
@@ -34,19 +34,21 @@ struct cp_cts {
   __device__ __forceinline__ cp_gi *at(int i) const { return table ? table[i] : base + i * stride; }
 };
 
-// blockIdx.y = pair * 4 + which (0: c0, 1: c1, 2: d0, 3: d1)
+// blockIdx.y = pair * 4 + which (0: c0, 1: c1, 2: d0, 3: d1).  X holds
+// `xrows` residue polynomials per input; the B residues go to rows
+// boff .. boff + kl - 1, the Q residues (copy_q) to rows 0 .. k - 1.
 __global__ void __launch_bounds__(256)
     k_ext_q_to_qb(cp_gi *__restrict__ X, cp_cts ct1, cp_cts ct2, int ng,
-                  const cp_hps *__restrict__ H, bool copy_q) {
+                  const cp_hps *__restrict__ H, int xrows, int boff, bool copy_q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
-  const int k = H->k, kl = H->kl;
+  const int k = H->k;
   const int pair = blockIdx.y >> 2, which = blockIdx.y & 3;
   const cp_gi *in = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)(which & 1) * k * ng;
-  cp_gi *out = X + (size_t)blockIdx.y * (k + kl) * ng;
+  cp_gi *out = X + (size_t)blockIdx.y * xrows * ng;
   if (copy_q)  // else the forward transform reads the Q rows from the input itself
     for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
-  hps_q_to_b_slot(in, ng, out + (size_t)k * ng, ng, c, H);
+  hps_q_to_b_slot(in, ng, out + (size_t)boff * ng, ng, c, H);
 }
 
 // Y[pair][m][r][slot] from X[pair][0..3][r][slot]; nres = k + kl residues.
@@ -73,55 +75,113 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Complex scaling t/Q, residue i of one slot (cudaengine.cu:401-433): the
+// reference forms
+//     y_i = sum_j (x_j w_ij mod b_i  (+)  h_j)  (+)  round  (+)  lambda_i xB_i mod b_i
+// with its single-correction addmod (+), where h_j = floor(x_j omega_frac_j
+// / 2^128) < q_j and round = llrint(sum_j RN(l_j 2^-64)) <= k.  When every
+// h_j < b_i, every operand of the chain is below b_i, the chain is plain
+// modular addition, and the exact sum reduced once (a 160-bit accumulation,
+// hsum = sum_j h_j + round added as its starting value) is the same word.
+// An h_j >= b_i needs x_j within 2^30 of q_j (the primes lie within 2^30 of
+// 2^63): about one coefficient in 2^33.  Such a slot takes the reference's
+// chain as written (`slow`), so the result is the same word in every case.
+__device__ __forceinline__ uint64_t scale_exact(const cp_acc &hsum, const uint64_t *x, int kq,
+                                                const uint64_t (*wint)[2], uint64_t xb,
+                                                const uint64_t (&lam)[2], uint64_t b) {
+  cp_acc a = hsum;
+#pragma unroll
+  for (int j = 0; j < kq; j++) cp_acc_mac(a, x[j], wint[j][0]);
+  cp_acc_mac(a, xb, lam[0]);
+  return cp_acc_reduce(a, b);
+}
+
+// hsum = sum_j h_j + round as a 160-bit accumulator value
+__device__ __forceinline__ void hsum_add(cp_acc &s, uint64_t v) {
+  unsigned c = 0;
+  s.lo = cp_addc64(s.lo, v, c);
+  s.hi += c;
+}
+
 // Complex scaling t/Q (QB -> B) and B -> Q in registers, K = k known at
 // compile time.  blockIdx.y = output polynomial index (pair * 3 + m).  The
 // fraction products x_j omega_frac_j and their FP64 sum do not depend on the
 // B residue i and are formed once; the B residues are produced one at a time
 // and folded straight into the B -> Q sums (ascending i, as the reference's
-// loops), so only K-sized arrays stay live.  Same operations, same order and
-// therefore the same values as hps_scale_slot followed by hps_b_to_q_slot.
+// loops).  B -> Q keeps one exact 160-bit sum per output residue for K <= 4
+// and one modular word for larger K (K 160-bit sums would stay live across
+// the whole B-residue loop: 272 VGPRs at K = 8); both give the exact residue,
+// the reference's chain of reduced terms.
 template <int K>
 __global__ void __launch_bounds__(256)
-    k_scale_to_q(cp_cts out, const cp_gi *__restrict__ Y, int ng, const cp_hps *__restrict__ H) {
+    k_scale_to_q(cp_cts out, const cp_gi *__restrict__ Y, int ng, const cp_hps *__restrict__ H,
+                 bool chain) {
+  constexpr bool kAcc = K <= 4;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int kl = H->kl;
   const int pair = blockIdx.y / 3, m = blockIdx.y % 3;
   const cp_gi *y = Y + (size_t)blockIdx.y * (K + kl) * ng;
   const cp_gi *yb = y + (size_t)K * ng;
-  cp_gi x[K];
+  uint64_t xr[K], xi[K];
   uint64_t fr[K], fi[K];
+  uint64_t bmin = H->b[0].p;
+  for (int i = 1; i < kl; i++) bmin = H->b[i].p < bmin ? H->b[i].p : bmin;
   const double two_m64 = 0x1p-64;
   double acc_r = 0.0, acc_i = 0.0;
+  bool slow = chain;
+  cp_acc hr, hi;
+  cp_acc_init(hr);
+  cp_acc_init(hi);
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    x[j] = y[(size_t)j * ng + c];
+    const cp_gi v = y[(size_t)j * ng + c];
+    xr[j] = v.re;
+    xi[j] = v.imag;
     uint64_t lr, li;
-    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, fr[j]);
-    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, fi[j]);
-    acc_r = fma((double)lr, two_m64, acc_r);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], xr[j], lr, fr[j]);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], xi[j], li, fi[j]);
+    acc_r = fma((double)lr, two_m64, acc_r);  // __fma_rn, ascending j
     acc_i = fma((double)li, two_m64, acc_i);
+    slow |= (fr[j] >= bmin) | (fi[j] >= bmin);
+    hsum_add(hr, fr[j]);
+    hsum_add(hi, fi[j]);
   }
   const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
-  // B -> Q sums, one modular word per output residue: K 160-bit accumulators
-  // would stay live across the whole B-residue loop (272 VGPRs at K = 8)
-  uint64_t sr[K], si[K];
+  hsum_add(hr, round_r);
+  hsum_add(hi, round_i);
+  cp_acc ar_acc[kAcc ? K : 1], ai_acc[kAcc ? K : 1];
+  uint64_t sr[kAcc ? 1 : K], si[kAcc ? 1 : K];
 #pragma unroll
-  for (int j = 0; j < K; j++) sr[j] = si[j] = 0;
+  for (int j = 0; j < K; j++) {
+    if constexpr (kAcc) {
+      cp_acc_init(ar_acc[j]);
+      cp_acc_init(ai_acc[j]);
+    } else {
+      sr[j] = si[j] = 0;
+    }
+  }
   double vr = 0.0, vi = 0.0;
   for (int i = 0; i < kl; i++) {
     const uint64_t b = H->b[i].p;
-    uint64_t rr = 0, ri = 0;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      rr = addmod_ref(rr, addmod_ref(shoup2(x[j].re, H->wint[i][j], b), fr[j], b), b);
-      ri = addmod_ref(ri, addmod_ref(shoup2(x[j].imag, H->wint[i][j], b), fi[j], b), b);
-    }
-    rr = addmod_ref(rr, round_r, b);
-    ri = addmod_ref(ri, round_i, b);
     const cp_gi yv = yb[(size_t)i * ng + c];
-    rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
-    ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+    uint64_t rr, ri;
+    if (!slow) {
+      rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
+      ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
+    } else {  // the reference's chain, operation for operation
+      rr = 0;
+      ri = 0;
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        rr = addmod_ref(rr, addmod_ref(shoup2(xr[j], H->wint[i][j], b), fr[j], b), b);
+        ri = addmod_ref(ri, addmod_ref(shoup2(xi[j], H->wint[i][j], b), fi[j], b), b);
+      }
+      rr = addmod_ref(rr, round_r, b);
+      ri = addmod_ref(ri, round_i, b);
+      rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
+      ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+    }
     // B -> Q, term i (cudaengine.cu:688-703)
     const uint64_t ar = shoup2(rr, H->invBbi[i], b), ai = shoup2(ri, H->invBbi[i], b);
     const double bd = (double)b;
@@ -129,9 +189,14 @@ __global__ void __launch_bounds__(256)
     vi = vi + (double)ai / bd;
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      const uint64_t q = H->q[j].p;
-      sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
-      si[j] = addmod_ref(si[j], shoup2(ai, H->Bbiqi[j][i], q), q);
+      if constexpr (kAcc) {
+        cp_acc_mac(ar_acc[j], ar, H->Bbiqi[j][i][0]);
+        cp_acc_mac(ai_acc[j], ai, H->Bbiqi[j][i][0]);
+      } else {
+        const uint64_t q = H->q[j].p;
+        sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
+        si[j] = addmod_ref(si[j], shoup2(ai, H->Bbiqi[j][i], q), q);
+      }
     }
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
@@ -140,62 +205,174 @@ __global__ void __launch_bounds__(256)
   for (int j = 0; j < K; j++) {
     const uint64_t q = H->q[j].p;
     cp_gi r;
-    r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
-    r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
+    if constexpr (kAcc) {
+      const uint64_t bq = H->Bqi[j][0], nbq = bq ? q - bq : 0;  // -B mod q_j
+      cp_acc_mac(ar_acc[j], v_re, nbq);
+      cp_acc_mac(ai_acc[j], v_im, nbq);
+      r.re = cp_acc_reduce(ar_acc[j], q);
+      r.imag = cp_acc_reduce(ai_acc[j], q);
+    } else {
+      r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
+      r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
+    }
     o[(size_t)j * ng + c] = r;
   }
 }
 
-// Forward DGT of the pipeline's extended operands (n <= 2^14), spectrum kept
-// in thread-major order: slot i of thread t of a residue polynomial goes to
-// X[unit][i * T + t], where the inverse below reads it back into the same
-// register of the same thread (its first inverse pass starts in the forward's
-// last-pass layout).  X is private to the pipeline, so neither side needs the
-// LDS exchange that puts a spectrum into position order, and every global
-// access is lane-contiguous.
+// ---- n <= 2^14: every transform of one (pair, residue) in one workgroup ----
+// The four forward transforms (c0, c1, d0, d1), the slot-wise tensor product
+// and the three inverse transforms of residue r of one ciphertext pair run in
+// one workgroup, so the spectra never make a full trip through HBM and each
+// workgroup's load and store phases are spread over seven transforms.  The
+// registers hold one spectrum beside the one being transformed, as in the
+// fused product kernel; the two further spectra the product needs wait in
+// the pair's own y1 and y2 output rows, addressed thread-major (slot i of
+// thread t at i * T + t: coalesced, and read back only by the thread that
+// wrote it):
+//   S_c0 = F(c0)                      held
+//   S_d0 = F(d0)         -> y1 park
+//   y0 = I(S_c0 S_d0)
+//   S_d1 = F(d1);  S_c0 S_d1 -> y2 park;  S_d1 held
+//   S_c1 = F(c1)
+//   y1 = I(S_c0 S_d1 + S_c1 S_d0),  y2 = I(S_c1 S_d1)
+// The final y1 and y2 stores overwrite the parks in another layout; every
+// thread has read its parks before the first workgroup barrier of the y1
+// inverse (each inverse pass plan of 2^10 .. 2^13 slots has at least one), so
+// no store can overtake a park read.  Workgroups run residue-major (all pairs
+// of residue 0, then residue 1, ...), so consecutive workgroups share one
+// residue's twiddle and twist tables in L2.
+// Wave-uniform pointer: kept in SGPRs (the compiler otherwise holds some
+// uniform bases in VGPR pairs across the transforms, where they spill).
+template <class P>
+__device__ __forceinline__ P *uniform_ptr(P *q) {
+  const uint64_t v = (uint64_t)q;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (P *)(((uint64_t)hi << 32) | lo);
+}
+// An opaque copy of the thread index: the address arithmetic that uses it
+// cannot be hoisted above this point (or shared between transforms and held
+// in VGPRs across them).
+__device__ __forceinline__ int opq(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+template <int LOGN>
+__device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restrict__ src, int t,
+                                         const uint64_t *__restrict__ tsf, uint64_t p) {
+  t = opq(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = src[first_index<LOGN, 0, LOGN>(t, i)];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    twist(x[i], tsf, first_index<LOGN, 0, LOGN>(t, i), p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+template <int LOGN>
+__device__ __forceinline__ void fti_store(cp_gi *__restrict__ dst, cp_gi (&x)[EPT], int t,
+                                          const uint64_t *__restrict__ tsi, uint64_t p,
+                                          bool active) {
+  t = opq(t);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int g = first_index<LOGN, 0, LOGN>(t, i);
+      twist(x[i], tsi, g, p);
+      dst[g] = x[i];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
-    k_dgt_fwd_tm(cp_gi *__restrict__ X, int nunits, int nres, int k, cp_cts ct1, cp_cts ct2,
-                 const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                 const uint64_t *__restrict__ tsf_all) {
+    k_hom_fti(cp_gi *__restrict__ Y, const cp_gi *__restrict__ XB, int batch, int nres, int k,
+              cp_cts ct1, cp_cts ct2, const cp_prime *__restrict__ primes,
+              const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ twi_all,
+              const uint64_t *__restrict__ tsf_all, const uint64_t *__restrict__ tsi_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
   const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
   cp_gi *lds = lds_all + row * lds_row<LOGN>();
-  // workgroups run residue-major (all units of residue 0, then residue 1, ...)
-  // so that consecutive workgroups share one residue's tables in L2
+  const int nunits = batch * nres;
   int u = blockIdx.x * ROWS + row;
   const bool active = u < nunits;
   if (!active) u = nunits - 1;
-  const int per_r = nunits / nres;
-  int rid = u / per_r;
+  int rid = u / batch;
   if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
-  const int unit = (u - rid * per_r) * nres + rid;
-  const uint64_t p = primes[rid].p;
-  const uint64_t *tw = twf_all + (size_t)rid * N, *tt = tsf_all + (size_t)rid * N * 6;
-  cp_gi *poly = X + (size_t)unit * N;
+  const int pair = u - rid * batch;
+  const cp_prime pr = primes[rid];
+  const uint64_t p = pr.p;
+  const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
+  const uint64_t *tsf = tsf_all + (size_t)rid * N * 6, *tsi = tsi_all + (size_t)rid * N * 6;
   // a Q residue comes straight from its ciphertext, a B residue from the
-  // extension written by k_ext_q_to_qb (unit = (pair * 4 + which) * nres + r)
-  const cp_gi *src = poly;
+  // extension (XB = [pair][which][kl][N])
+  const cp_gi *c0, *c1, *d0, *d1;
   if (rid < k) {
-    const int pw = unit / nres, pair = pw >> 2, which = pw & 3;
-    src = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)((which & 1) * k + rid) * N;
+    c0 = ct1.at(pair) + (size_t)rid * N;
+    c1 = c0 + (size_t)k * N;
+    d0 = ct2.at(pair) + (size_t)rid * N;
+    d1 = d0 + (size_t)k * N;
+  } else {
+    const size_t kn = (size_t)(nres - k) * N;
+    c0 = XB + (size_t)pair * 4 * kn + (size_t)(rid - k) * N;
+    c1 = c0 + kn;
+    d0 = c1 + kn;
+    d1 = d0 + kn;
   }
-  cp_gi x[EPT];
+  cp_gi *y0 = Y + ((size_t)pair * 3 * nres + rid) * N;
+  cp_gi *y1 = y0 + (size_t)nres * N, *y2 = y1 + (size_t)nres * N;
+  if (ROWS == 1) {
+    c0 = uniform_ptr(c0);
+    c1 = uniform_ptr(c1);
+    d0 = uniform_ptr(d0);
+    d1 = uniform_ptr(d1);
+    y0 = uniform_ptr(y0);
+    y1 = uniform_ptr(y1);
+    y2 = uniform_ptr(y2);
+  }
   MapIdentity id;
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    const int g = first_index<LOGN, 0, LOGN>(t, i);
-    x[i] = src[g];
-    twist(x[i], tt, g, p);
-  }
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, tw, p, id);
-  // every thread has loaded its inputs before the first exchange barrier
+  cp_gi xa[EPT], xb[EPT];
+  fti_load<LOGN>(xa, c0, t, tsf, p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twf, p, id);  // S_c0, held
+  fti_load<LOGN>(xb, d0, t, tsf, p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twf, p, id);  // S_d0
   if (active) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) poly[i * T + t] = x[i];
+    for (int i = 0; i < EPT; i++) y1[i * T + opq(t)] = xb[i];
   }
+#pragma unroll
+  for (int i = 0; i < EPT; i++) xb[i] = cp_gimul(xa[i], xb[i], pr);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twi, p, id);
+  fti_store<LOGN>(y0, xb, t, tsi, p, active);
+  fti_load<LOGN>(xb, d1, t, tsf, p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twf, p, id);  // S_d1, held
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const cp_gi w = cp_gimul(xa[i], xb[i], pr);
+    if (active) y2[i * T + opq(t)] = w;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  fti_load<LOGN>(xa, c1, t, tsf, p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twf, p, id);  // S_c1
+  const int tp = opq(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    // one slot's park loads in flight at a time (all 16 hoisted would spill)
+    const cp_gi pd0 = y1[i * T + tp], pc0d1 = y2[i * T + tp];
+    const cp_gi cc = xa[i];
+    xb[i] = cp_gimul(cc, xb[i], pr);
+    const cp_gi w = cp_gimul(cc, pd0, pr);
+    xa[i].re = cp_add(w.re, pc0d1.re, p);
+    xa[i].imag = cp_add(w.imag, pc0d1.imag, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twi, p, id);
+  fti_store<LOGN>(y1, xa, t, tsi, p, active);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twi, p, id);
+  fti_store<LOGN>(y2, xb, t, tsi, p, active);
 }
 
 // ---- small batches: one thread per (slot, output residue) ------------------
@@ -209,13 +386,13 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 // Q -> B, output residue i = blockIdx.z (blockIdx.y = pair * 4 + which)
 __global__ void __launch_bounds__(256)
     k_ext_q_to_b_wide(cp_gi *__restrict__ X, cp_cts ct1, cp_cts ct2, int ng,
-                      const cp_hps *__restrict__ H, bool copy_q) {
+                      const cp_hps *__restrict__ H, int xrows, int boff, bool copy_q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
-  const int k = H->k, kl = H->kl, i = blockIdx.z;
+  const int k = H->k, i = blockIdx.z;
   const int pair = blockIdx.y >> 2, which = blockIdx.y & 3;
   const cp_gi *in = (which < 2 ? ct1.at(pair) : ct2.at(pair)) + (size_t)(which & 1) * k * ng;
-  cp_gi *out = X + (size_t)blockIdx.y * (k + kl) * ng;
+  cp_gi *out = X + (size_t)blockIdx.y * xrows * ng;
   if (copy_q && i == 0)
     for (int j = 0; j < k; j++) out[(size_t)j * ng + c] = in[(size_t)j * ng + c];
   uint64_t ar[CP_MAX_BASE], ai[CP_MAX_BASE];
@@ -253,7 +430,7 @@ __global__ void __launch_bounds__(256)
   }
   cp_acc_mac(sr, v_re, nqb);
   cp_acc_mac(si, v_im, nqb);
-  out[(size_t)(k + i) * ng + c] = cp_gi{cp_acc_reduce(sr, b), cp_acc_reduce(si, b)};
+  out[(size_t)(boff + i) * ng + c] = cp_gi{cp_acc_reduce(sr, b), cp_acc_reduce(si, b)};
 }
 
 // t/Q scaling, B residue i = blockIdx.z of output polynomial blockIdx.y, and
@@ -261,37 +438,55 @@ __global__ void __launch_bounds__(256)
 template <int K>
 __global__ void __launch_bounds__(256)
     k_scale_b_wide(cp_gi *__restrict__ Z, const cp_gi *__restrict__ Y, int ng,
-                   const cp_hps *__restrict__ H) {
+                   const cp_hps *__restrict__ H, bool chain) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ng) return;
   const int kl = H->kl, i = blockIdx.z, pm = blockIdx.y;
   const cp_gi *y = Y + (size_t)pm * (K + kl) * ng;
-  cp_gi x[K];
+  const uint64_t b = H->b[i].p;
+  uint64_t xr[K], xi[K];
   uint64_t fr[K], fi[K];
   const double two_m64 = 0x1p-64;
   double acc_r = 0.0, acc_i = 0.0;
+  bool slow = chain;
+  cp_acc hr, hi;
+  cp_acc_init(hr);
+  cp_acc_init(hi);
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    x[j] = y[(size_t)j * ng + c];
+    const cp_gi v = y[(size_t)j * ng + c];
+    xr[j] = v.re;
+    xi[j] = v.imag;
     uint64_t lr, li;
-    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, fr[j]);
-    umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, fi[j]);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], xr[j], lr, fr[j]);
+    umul128hi(H->wfrac[j][0], H->wfrac[j][1], xi[j], li, fi[j]);
     acc_r = fma((double)lr, two_m64, acc_r);
     acc_i = fma((double)li, two_m64, acc_i);
+    slow |= (fr[j] >= b) | (fi[j] >= b);
+    hsum_add(hr, fr[j]);
+    hsum_add(hi, fi[j]);
   }
   const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
-  const uint64_t b = H->b[i].p;
-  uint64_t rr = 0, ri = 0;
-#pragma unroll
-  for (int j = 0; j < K; j++) {
-    rr = addmod_ref(rr, addmod_ref(shoup2(x[j].re, H->wint[i][j], b), fr[j], b), b);
-    ri = addmod_ref(ri, addmod_ref(shoup2(x[j].imag, H->wint[i][j], b), fi[j], b), b);
-  }
-  rr = addmod_ref(rr, round_r, b);
-  ri = addmod_ref(ri, round_i, b);
+  hsum_add(hr, round_r);
+  hsum_add(hi, round_i);
   const cp_gi yv = y[(size_t)(K + i) * ng + c];
-  rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
-  ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+  uint64_t rr, ri;
+  if (!slow) {  // same word as the reference's chain (scale_exact)
+    rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
+    ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
+  } else {
+    rr = 0;
+    ri = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      rr = addmod_ref(rr, addmod_ref(shoup2(xr[j], H->wint[i][j], b), fr[j], b), b);
+      ri = addmod_ref(ri, addmod_ref(shoup2(xi[j], H->wint[i][j], b), fi[j], b), b);
+    }
+    rr = addmod_ref(rr, round_r, b);
+    ri = addmod_ref(ri, round_i, b);
+    rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
+    ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+  }
   Z[((size_t)pm * kl + i) * ng + c] = cp_gi{shoup2(rr, H->invBbi[i], b), shoup2(ri, H->invBbi[i], b)};
 }
 
@@ -325,60 +520,6 @@ __global__ void __launch_bounds__(256)
   r.re = cp_acc_reduce(sr, q);
   r.imag = cp_acc_reduce(si, q);
   out.at(pair)[((size_t)m * k + j) * ng + c] = r;
-}
-
-// Tensor product fused into the inverse transform (n <= 2^14): one
-// workgroup row per output residue polynomial (pair, m, r) reads the
-// thread-major spectra X[pair][0..3][r] written by k_dgt_fwd_tm, forms
-// m = 0: c0 d0,  m = 1: c0 d1 + c1 d0,  m = 2: c1 d1
-// slot-wise in registers, then runs the inverse DGT and untwist.
-template <int LOGN, int ROWS>
-__global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
-    k_tensor_idgt(cp_gi *__restrict__ Y, const cp_gi *__restrict__ X, int nunits, int nres,
-                  const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twi_all,
-                  const uint64_t *__restrict__ tsi_all) {
-  constexpr int N = 1 << LOGN, T = N / EPT;
-  extern __shared__ cp_gi lds_all[];
-  const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
-  const int t = ROWS == 1 ? (int)threadIdx.x : (int)(threadIdx.x % T);
-  cp_gi *lds = lds_all + row * lds_row<LOGN>();
-  // residue-major workgroup order, as in k_dgt_fwd_tm
-  int u = blockIdx.x * ROWS + row;
-  const bool active = u < nunits;
-  if (!active) u = nunits - 1;
-  const int per_r = nunits / nres;
-  int rid = u / per_r;
-  if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);
-  const int pm = u - rid * per_r, unit = pm * nres + rid;  // (pair * 3 + m) * nres + r
-  const int pair = pm / 3, m = pm % 3;
-  const cp_prime pr = primes[rid];
-  const uint64_t p = pr.p;
-  const size_t per_pair = (size_t)nres * N;
-  const cp_gi *x = X + (size_t)pair * 4 * per_pair + (size_t)rid * N;
-  const cp_gi *xa = x + (m == 2 ? per_pair : 0), *xb = x + (m == 0 ? 2 : 3) * per_pair;
-  cp_gi v[EPT];
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    const int g = i * T + t;  // thread-major
-    v[i] = cp_gimul(xa[g], xb[g], pr);
-    if (m == 1) {  // + c1 d0
-      const cp_gi w = cp_gimul(x[per_pair + g], x[2 * per_pair + g], pr);
-      v[i].re = cp_add(v[i].re, w.re, p);
-      v[i].imag = cp_add(v[i].imag, w.imag, p);
-    }
-  }
-  const uint64_t *twi = twi_all + (size_t)rid * N, *tsi = tsi_all + (size_t)rid * N * 6;
-  MapIdentity id;
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(v, lds, t, twi, p, id);
-  if (active) {
-    cp_gi *y = Y + (size_t)unit * N;
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int g = first_index<LOGN, 0, LOGN>(t, i);
-      twist(v[i], tsi, g, p);
-      y[g] = v[i];
-    }
-  }
 }
 
 // ---- n = 2^15: the two kernels above for 16384-slot residue polynomials --
@@ -464,45 +605,26 @@ __global__ void __launch_bounds__(1024)
 }
 
 template <int LOGN>
-static int launch_fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
+static int launch_fti(cupoly_ctx *c, cp_gi *Y, const cp_gi *XB, int batch, cp_cts ct1,
+                      cp_cts ct2) {
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   cupoly_engine *e = c->e;
-  hipLaunchKernelGGL((k_dgt_fwd_tm<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
-                     c->stream, X, nunits, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
-                     e->d_twist_fwd);
+  const int nunits = batch * e->nres;
+  hipLaunchKernelGGL((k_hom_fti<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
+                     c->stream, Y, XB, batch, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
+                     e->d_tw_inv, e->d_twist_fwd, e->d_twist_inv);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }
 
-static int fwd_tm(cupoly_ctx *c, cp_gi *X, int nunits, cp_cts ct1, cp_cts ct2) {
+static int fti(cupoly_ctx *c, cp_gi *Y, const cp_gi *XB, int batch, cp_cts ct1, cp_cts ct2) {
   switch (c->e->logng) {
-    case 10: return launch_fwd_tm<10>(c, X, nunits, ct1, ct2);
-    case 11: return launch_fwd_tm<11>(c, X, nunits, ct1, ct2);
-    case 12: return launch_fwd_tm<12>(c, X, nunits, ct1, ct2);
-    case 13: return launch_fwd_tm<13>(c, X, nunits, ct1, ct2);
-    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fwd_tm: ring degree");
-  }
-}
-
-template <int LOGN>
-static int launch_tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
-  constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
-  const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
-  cupoly_engine *e = c->e;
-  hipLaunchKernelGGL((k_tensor_idgt<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
-                     c->stream, Y, X, nunits, e->nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
-  CP_LAUNCH_CHECK();
-  return CUPOLY_OK;
-}
-
-static int tensor_idgt(cupoly_ctx *c, cp_gi *Y, const cp_gi *X, int nunits) {
-  switch (c->e->logng) {
-    case 10: return launch_tensor_idgt<10>(c, Y, X, nunits);
-    case 11: return launch_tensor_idgt<11>(c, Y, X, nunits);
-    case 12: return launch_tensor_idgt<12>(c, Y, X, nunits);
-    case 13: return launch_tensor_idgt<13>(c, Y, X, nunits);
-    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "tensor_idgt: ring degree");
+    case 10: return launch_fti<10>(c, Y, XB, batch, ct1, ct2);
+    case 11: return launch_fti<11>(c, Y, XB, batch, ct1, ct2);
+    case 12: return launch_fti<12>(c, Y, XB, batch, ct1, ct2);
+    case 13: return launch_fti<13>(c, Y, XB, batch, ct1, ct2);
+    default: return cp_fail(CUPOLY_ERR_UNSUPPORTED, "hom_mul: ring degree");
   }
 }
 
@@ -512,9 +634,7 @@ template <int LOGN>
 static int homomul_setup_t() {
   constexpr int R = rows_for<LOGN>();
   const int lds = (int)((size_t)R * lds_row<LOGN>() * sizeof(cp_gi));
-  CP_HIP(hipFuncSetAttribute((const void *)k_dgt_fwd_tm<LOGN, R>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  CP_HIP(hipFuncSetAttribute((const void *)k_tensor_idgt<LOGN, R>,
+  CP_HIP(hipFuncSetAttribute((const void *)k_hom_fti<LOGN, R>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   return CUPOLY_OK;
 }
@@ -541,7 +661,7 @@ template <int K>
 static void launch_scale(cupoly_ctx *c, cp_cts out, const cp_gi *Y, int batch) {
   const int ng = c->e->ng;
   hipLaunchKernelGGL(k_scale_to_q<K>, dim3((ng + 255) / 256, batch * 3), dim3(256), 0, c->stream,
-                     out, Y, ng, c->e->d_hps);
+                     out, Y, ng, c->e->d_hps, c->scale_chain);
 }
 
 template <int K>
@@ -549,7 +669,7 @@ static void launch_scale_wide(cupoly_ctx *c, cp_cts out, const cp_gi *Y, cp_gi *
   const int ng = c->e->ng, kl = c->e->kl;
   const dim3 g((ng + 255) / 256, batch * 3);
   hipLaunchKernelGGL(k_scale_b_wide<K>, dim3(g.x, g.y, kl), dim3(256), 0, c->stream, Z, Y, ng,
-                     c->e->d_hps);
+                     c->e->d_hps, c->scale_chain);
   hipLaunchKernelGGL(k_b_to_q_wide, dim3(g.x, g.y, K), dim3(256), 0, c->stream, out, Z, ng, K,
                      c->e->d_hps);
 }
@@ -639,37 +759,51 @@ static int grow_ptr_tables(cupoly_ctx *c, size_t need) {
 }
 
 // (k + kl)-residue scratch of the pipeline: 4 extended inputs and 3 products per pair.
+// n = 2^11 .. 2^14: one workgroup runs every transform of a (pair, residue)
+static bool hom_mul_fti_sizes(const cupoly_engine *e) { return e->logng >= 10 && e->logng <= 13; }
+
+// Scratch of the pipeline per pair: the extended inputs X (4 polynomials of
+// the B rows only when the fused transforms read the Q rows from the
+// ciphertexts, of all k + kl rows otherwise) and the 3 products Y (k + kl
+// rows), N Gaussian slots each.
 static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
-  return (size_t)batch * 7 * (e->k + e->kl) * e->ng * sizeof(cp_gi);
+  const size_t xrows = hom_mul_fti_sizes(e) ? e->kl : e->k + e->kl;
+  return (size_t)batch * (4 * xrows + 3 * (size_t)(e->k + e->kl)) * e->ng * sizeof(cp_gi);
 }
 
-static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
+// The pipeline on the context's stream with scratch `ws` of at least
+// hom_mul_ws_bytes(e, batch) bytes; allocates nothing.
+static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch, void *ws) {
   cupoly_engine *e = c->e;
   const int k = e->k, kl = e->kl, nres = k + kl, ng = e->ng;
-  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
-  const size_t xs = (size_t)batch * 4 * nres * ng;
-  void *ws;
-  int rc = cp_workspace(c, hom_mul_ws_bytes(e, batch), &ws);
-  if (rc) return rc;
+  const bool fti_sizes = hom_mul_fti_sizes(e);
+  const int xrows = fti_sizes ? kl : nres;
+  const size_t xs = (size_t)batch * 4 * xrows * ng;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;
   const dim3 sg((ng + 255) / 256);
+  int rc;
   // n = 2^15 uses the one-workgroup-per-polynomial kernels once the batch has
   // forward residue polynomials for three quarters of the CUs; smaller
   // batches keep the split transforms, which put more workgroups on each
   // polynomial (measured at k = 8: 2 pairs split 0.24 ms against 0.26 ms,
   // 3 pairs 0.32 against 0.28 ms)
   const bool big16 = e->logng == 14 && 4 * batch * 4 * nres >= 3 * e->ncu;
-  const bool fused_sizes = (e->logng >= 10 && e->logng <= 13) || big16;
+  // the Q rows are copied into X only for the generic transforms; the fused
+  // ones read them from the ciphertexts
+  const bool copy_q = !(fti_sizes || big16);
+  const int boff = fti_sizes ? 0 : k;
   // per-residue threads while the per-slot forms would leave the GPU mostly idle
   const bool wide = (long)batch * 3 * ng < c->wide_slots;
   if (wide)
     hipLaunchKernelGGL(k_ext_q_to_b_wide, dim3(sg.x, batch * 4, kl), dim3(256), 0, c->stream, X,
-                       ct1, ct2, ng, e->d_hps, !fused_sizes);
+                       ct1, ct2, ng, e->d_hps, xrows, boff, copy_q);
   else
     hipLaunchKernelGGL(k_ext_q_to_qb, dim3(sg.x, batch * 4), dim3(256), 0, c->stream, X, ct1,
-                       ct2, ng, e->d_hps, !fused_sizes);
+                       ct2, ng, e->d_hps, xrows, boff, copy_q);
   CP_LAUNCH_CHECK();
-  if (big16) {
+  if (fti_sizes) {
+    if ((rc = fti(c, Y, X, batch, ct1, ct2))) return rc;
+  } else if (big16) {
     const size_t lds = lds_row<13>() * sizeof(cp_gi);
     hipLaunchKernelGGL(k_dgt_fwd_tm16, dim3(batch * 4 * nres), dim3(1024), lds, c->stream, X,
                        batch * 4 * nres, nres, k, ct1, ct2, e->d_primes, e->d_tw_fwd,
@@ -678,9 +812,6 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     hipLaunchKernelGGL(k_tensor_idgt16, dim3(batch * 3 * nres), dim3(1024), lds, c->stream, Y, X,
                        batch * 3 * nres, nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
     CP_LAUNCH_CHECK();
-  } else if (fused_sizes) {
-    if ((rc = fwd_tm(c, X, batch * 4 * nres, ct1, ct2))) return rc;
-    if ((rc = tensor_idgt(c, Y, X, batch * 3 * nres))) return rc;
   } else {
     if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
     const long total = (long)batch * nres * ng;
@@ -692,7 +823,16 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     if ((rc = cp_dgt_launch(c, Y, 0, nres, batch * 3, CUPOLY_INVERSE))) return rc;
   }
   // X is dead once the products are formed: it holds the B -> Q operands
+  // (batch * 3 * kl <= batch * 4 * xrows residue polynomials)
   return wide ? scale_to_q_wide(c, out, Y, X, batch) : scale_to_q(c, out, Y, batch);
+}
+
+static int hom_mul_alloc_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
+  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
+  void *ws;
+  int rc = cp_workspace(c, hom_mul_ws_bytes(c->e, batch), &ws);
+  if (rc) return rc;
+  return hom_mul_run(c, out, ct1, ct2, batch, ws);
 }
 
 extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
@@ -703,7 +843,7 @@ extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
   const size_t k = c->e->k, ng = c->e->ng;
   cp_cts o{(cp_gi *)out, 3 * k * ng, nullptr}, a{(cp_gi *)ct1, 2 * k * ng, nullptr},
       b{(cp_gi *)ct2, 2 * k * ng, nullptr};
-  return hom_mul_run(c, o, a, b, batch);
+  return hom_mul_alloc_run(c, o, a, b, batch);
 }
 
 extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
@@ -712,7 +852,7 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   if (c && batch == 0) return CUPOLY_OK;  // empty batch: tables may be null
   if (!c || !out || !ct1 || !ct2 || batch < 0)
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
-  // the limit hom_mul_run enforces, checked before the tables grow
+  // the limit hom_mul_alloc_run enforces, checked before the tables grow
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   for (int i = 0; i < batch; i++)
     if (!out[i] || !ct1[i] || !ct2[i]) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: null entry");
@@ -736,7 +876,7 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   c->ring_used[slot] = true;
   cp_gi **d = (cp_gi **)c->d_ptrs;
   cp_cts o{nullptr, 0, d}, a{nullptr, 0, d + batch}, b{nullptr, 0, d + 2 * batch};
-  return hom_mul_run(c, o, a, b, batch);
+  return hom_mul_alloc_run(c, o, a, b, batch);
 }
 
 extern "C" int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch) {
@@ -747,4 +887,19 @@ extern "C" int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch) {
   int rc = cp_workspace(c, hom_mul_ws_bytes(c->e, batch), &ws);
   if (rc) return rc;
   return grow_ptr_tables(c, (size_t)batch * 3 * sizeof(void *));
+}
+
+extern "C" int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain) {
+  if (!c || wide_slots < 0 || scale_chain < 0 || scale_chain > 1)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_set_tuning: bad argument");
+  c->wide_slots = wide_slots ? wide_slots : 1L << 16;
+  c->scale_chain = scale_chain != 0;
+  return CUPOLY_OK;
+}
+
+extern "C" int cupoly_hom_mul_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes) {
+  if (!e || !bytes || batch < 0 || batch > 65535 / 4)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_workspace_bytes: bad argument");
+  *bytes = hom_mul_ws_bytes(e, batch);
+  return CUPOLY_OK;
 }

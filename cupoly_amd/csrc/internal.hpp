@@ -87,6 +87,9 @@ struct cupoly_ctx {
   // hom-mul: below this many output slots (batch * 3 * n/2) the HPS steps run
   // one thread per (slot, residue) instead of one per slot
   long wide_slots = 1L << 16;
+  // hom-mul t/Q scaling: the reference's addmod chain on every slot instead of
+  // the exact sums (which give the same words)
+  bool scale_chain = false;
 };
 
 // error plumbing

@@ -207,8 +207,8 @@ int cupoly_polymul_plain(cupoly_ctx *c, cupoly_gi *out, cupoly_gi *a, cupoly_gi 
  * pair i: ct1[i] = (c0, c1), ct2[i] = (d0, d1), [batch][2][k][n/2] GIs, base Q,
  * coefficient domain; out[i] = [3][k][n/2]: round(t/Q (c0 d0, c0 d1 + c1 d0,
  * c1 d1)) mod Q, coefficient domain.  Uses a context-owned workspace of
- * batch * 7 * (k+kl) * n/2 GIs (allocated on first use, grown as needed).
- * batch <= 16383 (CUPOLY_ERR_ARG otherwise). */
+ * cupoly_hom_mul_workspace_bytes(e, batch) bytes (allocated on first use,
+ * grown as needed).  batch <= 16383 (CUPOLY_ERR_ARG otherwise). */
 int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, const cupoly_gi *ct2,
                          int batch);
 /* Same computation for gathered operands (a serving batch of requests whose
@@ -223,6 +223,19 @@ int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi 
  * the reference allocates its Context scratch once, context.cpp:19-67; a
  * serving loop calls this before it starts timing requests). */
 int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
+/* Launch choices of the hom-mul pipeline on this context (no reference
+ * counterpart): below `wide_slots` output slots (batch * 3 * n/2; 0 = the
+ * built-in 65536) the HPS steps run one thread per (slot, residue);
+ * scale_chain = 1 computes every t/Q residue with the reference's addmod
+ * chain (cudaengine.cu:401-433) instead of the exact sums, which give the
+ * same words (the default, 0, takes the chain only where the reference's
+ * operands arrive unreduced). */
+int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain);
+/* Device bytes of the hom-mul workspace for `batch` pairs of e's shape (no
+ * reference counterpart; the reference sizes its Context scratch from the
+ * ring shape, context.cpp:19-67): 16 * n/2 * batch * (4 kl + 3 (k + kl)) for
+ * n = 2^11 .. 2^14, 16 * n/2 * batch * 7 (k + kl) otherwise. */
+int cupoly_hom_mul_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes);
 
 /* Encryption-shaped step for a batch: out[i] = a[i] * s_i + e_i in the DGT
  * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian

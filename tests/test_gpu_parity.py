@@ -341,6 +341,36 @@ def test_hom_mul_batch_matches_oracle(gpu, k, n, batch):
         assert np.array_equal(h[i], O.hom_mul(P, ct1[i], ct2[i])), i
 
 
+@pytest.mark.parametrize("k,n,batch", [(4, 16384, 2), (2, 2048, 48), (3, 8192, 8),
+                                         (8, 32768, 3), (1, 4096, 5)])
+def test_hom_mul_scale_forms_agree(gpu, k, n, batch):
+    """The t/Q scaling computes each residue as an exact sum reduced once; the
+    reference's addmod chain (forced on every slot by scale_chain = 1) must give
+    the same words.  Both HPS launch forms (one thread per slot, and one per
+    (slot, residue), chosen through wide_slots) are run in both modes; all four
+    results are equal and one pair equals the oracle composition."""
+    import torch
+    e, ctx = engine(k, k + 1, n, 65537)
+    P = O.Params(k, k + 1, n, 65537)
+    ct1 = cp.to_device(rand_res(P, P.q, (batch, 2), 700 + k))
+    ct2 = cp.to_device(rand_res(P, P.q, (batch, 2), 710 + k))
+    outs = []
+    try:
+        for wide_slots in (1, 1 << 40):
+            for chain in (0, 1):
+                cp.hom_mul_set_tuning(ctx, wide_slots, chain)
+                o = cp.empty((batch, 3, k, P.Ng, 2))
+                cp.hom_mul_batch(ctx, o, ct1, ct2)
+                outs.append(o)
+    finally:
+        cp.hom_mul_set_tuning(ctx, 0, 0)
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    i = batch - 1
+    assert np.array_equal(cp.to_host(outs[0][i]),
+                          O.hom_mul(P, cp.to_host(ct1[i]), cp.to_host(ct2[i])))
+
+
 def test_hom_mul_full_batch_properties(gpu):
     """n = 2^14, k = 4, 128 pairs (the benchmarked batch): the tensor product is
     symmetric, so swapping the ciphertexts leaves every pair's result unchanged;
