@@ -72,6 +72,7 @@ SIGNATURES = {
     "cupoly_hom_mul_reserve": ([_vp, _int], _int),
     "cupoly_hom_mul_workspace_bytes": ([_vp, _int, _vp], _int),
     "cupoly_hom_mul_set_tuning": ([_vp, ctypes.c_long, _int], _int),
+    "cupoly_hom_mul_batch_noalloc": ([_vp, _vp, _vp, _vp, _int], _int),
     "cupoly_polymul_set_tuning": ([_vp, _int, _int], _int),
     "cupoly_polymul_autotune": ([_vp, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
@@ -367,6 +368,21 @@ def hom_mul_batch(ctx, out, ct1, ct2):
         if tuple(t.shape) != (batch, m, e.k, e.N, 2):
             raise ValueError("cupoly_amd: hom_mul_batch operand shape %s" % (tuple(t.shape),))
     _check(lib().cupoly_hom_mul_batch(ctx.handle, _ptr(out), _ptr(ct1), _ptr(ct2), batch))
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20
+
+
+def hom_mul_batch_noalloc(ctx, out, ct1, ct2):
+    """hom_mul_batch with the pipeline's scratch in the library's fixed
+    per-GPU area (NOALLOC_SCRATCH_BYTES): the call allocates nothing; batches
+    whose scratch exceeds the area raise.  Calls overlapping on one GPU race."""
+    e = ctx.engine
+    batch = ct1.shape[0]
+    for t, m in ((ct1, 2), (ct2, 2), (out, 3)):
+        if tuple(t.shape) != (batch, m, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: hom_mul_batch_noalloc operand shape %s" % (tuple(t.shape),))
+    _check(lib().cupoly_hom_mul_batch_noalloc(ctx.handle, _ptr(out), _ptr(ct1), _ptr(ct2), batch))
 
 
 def hom_mul_gather(ctx, outs, ct1s, ct2s):

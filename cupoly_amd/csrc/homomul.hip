@@ -22,6 +22,9 @@
 //   3. k_scale_to_q:    complex scaling t/Q (QB -> B) and B -> Q fused in
 //                       registers, one thread per slot, writing base Q.
 // Scratch lives in a grow-only workspace owned by the context.
+#include <mutex>
+#include <string>
+
 #include "dgt_dev.hpp"
 #include "hps_dev.hpp"
 
@@ -639,7 +642,39 @@ static int homomul_setup_t() {
   return CUPOLY_OK;
 }
 
+// ---- scratch of the allocation-free entry point ------------------------------
+// cupoly_hom_mul_batch_noalloc runs the pipeline with its scratch in this
+// file-scope device array: CUPOLY_NOALLOC_SCRATCH_BYTES on every GPU the
+// library runs on, held until the process ends.  In host code the name
+// `scratch` denotes a host-side stand-in, not device memory, so host code
+// never passes it to a kernel or a copy: the array's address on each device
+// is looked up once, with that device current (engine creation), by
+// hipGetSymbolAddress and kept in g_scratch_dev; every use goes through it.
+static __device__ __align__(256) unsigned char scratch[CUPOLY_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+
+static int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices)
+    return cp_fail(CUPOLY_ERR_UNSUPPORTED, "hom_mul scratch: device index beyond 63");
+  if (g_scratch_dev[device]) return CUPOLY_OK;
+  void *ptr = nullptr;
+  CP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return cp_fail(CUPOLY_ERR_HIP, "hom_mul scratch: no device address");
+  g_scratch_dev[device] = ptr;
+  return CUPOLY_OK;
+}
+
+static void *scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_dev[device] : nullptr;
+}
+
 int cp_homomul_setup(cupoly_engine *e) {
+  // runs with e's device current
+  if (int rc = scratch_setup(e->device)) return rc;
   switch (e->logng) {
     case 10: return homomul_setup_t<10>();
     case 11: return homomul_setup_t<11>();
@@ -902,4 +937,24 @@ extern "C" int cupoly_hom_mul_workspace_bytes(const cupoly_engine *e, int batch,
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_workspace_bytes: bad argument");
   *bytes = hom_mul_ws_bytes(e, batch);
   return CUPOLY_OK;
+}
+
+extern "C" int cupoly_hom_mul_batch_noalloc(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
+                                            const cupoly_gi *ct2, int batch) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: pointers may be null
+  if (!c || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch_noalloc: bad argument");
+  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch_noalloc: batch > 16383");
+  const size_t need = hom_mul_ws_bytes(c->e, batch);
+  if (need > (size_t)CUPOLY_NOALLOC_SCRATCH_BYTES)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_batch_noalloc: " + std::to_string(batch) +
+                                       " pairs need " + std::to_string(need) +
+                                       " bytes of scratch; the maximum is " +
+                                       std::to_string((size_t)CUPOLY_NOALLOC_SCRATCH_BYTES));
+  void *ws = scratch_of(c->e->device);
+  if (!ws) return cp_fail(CUPOLY_ERR_STATE, "hom_mul_batch_noalloc: no scratch on this device");
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_cts o{(cp_gi *)out, 3 * k * ng, nullptr}, a{(cp_gi *)ct1, 2 * k * ng, nullptr},
+      b{(cp_gi *)ct2, 2 * k * ng, nullptr};
+  return hom_mul_run(c, o, a, b, batch, ws);
 }

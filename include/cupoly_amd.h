@@ -223,6 +223,19 @@ int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi 
  * the reference allocates its Context scratch once, context.cpp:19-67; a
  * serving loop calls this before it starts timing requests). */
 int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
+/* cupoly_hom_mul_batch with no allocation of any kind (no reference
+ * counterpart: it serves callers that cannot allocate device memory, such as
+ * code under HIP graph capture or a tight serving loop).  The pipeline's
+ * scratch is a fixed device-resident area of CUPOLY_NOALLOC_SCRATCH_BYTES
+ * (128 MiB) per GPU, held from the first engine created on that GPU until the
+ * process ends; batches whose scratch (cupoly_hom_mul_workspace_bytes) exceeds
+ * it fail with CUPOLY_ERR_ARG (at n = 2^14, k = 4: 21 pairs).  Every call on
+ * one GPU shares that area: calls that overlap on one GPU (on different
+ * streams, or from different threads) race and must be serialised by the
+ * caller.  Same results as cupoly_hom_mul_batch. */
+#define CUPOLY_NOALLOC_SCRATCH_BYTES (128u << 20)
+int cupoly_hom_mul_batch_noalloc(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
+                                 const cupoly_gi *ct2, int batch);
 /* Launch choices of the hom-mul pipeline on this context (no reference
  * counterpart): below `wide_slots` output slots (batch * 3 * n/2; 0 = the
  * built-in 65536) the HPS steps run one thread per (slot, residue);
