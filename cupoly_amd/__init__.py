@@ -73,6 +73,14 @@ SIGNATURES = {
     "cupoly_hom_mul_workspace_bytes": ([_vp, _int, _vp], _int),
     "cupoly_hom_mul_set_tuning": ([_vp, ctypes.c_long, _int], _int),
     "cupoly_hom_mul_batch_noalloc": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_relin_key_create": ([_vp, _vp, _u64, _u64, ctypes.POINTER(_vp)], _int),
+    "cupoly_relin_key_destroy": ([_vp], _int),
+    "cupoly_relin_key_data": ([_vp, ctypes.POINTER(_vp)], _int),
+    "cupoly_relin_key_copy": ([_vp, _vp, _vp], _int),
+    "cupoly_relinearize": ([_vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_relin_batch": ([_vp, _vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_relin_gather": ([_vp, _vp, _vp, _vp, _vp, _int], _int),
+    "cupoly_hom_mul_relin_workspace_bytes": ([_vp, _int, _vp], _int),
     "cupoly_polymul_set_tuning": ([_vp, _int, _int], _int),
     "cupoly_polymul_autotune": ([_vp, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
@@ -368,6 +376,87 @@ def hom_mul_batch(ctx, out, ct1, ct2):
         if tuple(t.shape) != (batch, m, e.k, e.N, 2):
             raise ValueError("cupoly_amd: hom_mul_batch operand shape %s" % (tuple(t.shape),))
     _check(lib().cupoly_hom_mul_batch(ctx.handle, _ptr(out), _ptr(ct1), _ptr(ct2), batch))
+
+
+class RelinKey:
+    """Relinearization key for secret s ([k, N, 2] device tensor, base Q,
+    coefficient domain: residues of small integers), resident on the device
+    in the DGT domain as [k, 2, k, N, 2] (b_i, a_i) until destroy()."""
+
+    def __init__(self, ctx, s, seed, stream0=0):
+        e = ctx.engine
+        _need(s, e.k, e.N)
+        h = ctypes.c_void_p()
+        _check(lib().cupoly_relin_key_create(ctx.handle, _ptr(s), seed, stream0, ctypes.byref(h)))
+        self.handle, self.engine = h, e
+
+    def data(self):
+        """Device address of the key's [k, 2, k, N, 2] words."""
+        p = ctypes.c_void_p()
+        _check(lib().cupoly_relin_key_data(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def copy_to(self, ctx, dst):
+        """Copy the key's words into the device tensor dst [k, 2, k, N, 2] on
+        ctx's stream."""
+        e = self.engine
+        if tuple(dst.shape) != (e.k, 2, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: relin key copy target shape %s" % (tuple(dst.shape),))
+        _check(lib().cupoly_relin_key_copy(ctx.handle, self.handle, _ptr(dst)))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().cupoly_relin_key_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def relinearize(ctx, key, out, y):
+    """y [batch, 3, k, N, 2] (coefficient domain) -> out [batch, 2, k, N, 2]."""
+    e = ctx.engine
+    batch = y.shape[0]
+    if tuple(y.shape) != (batch, 3, e.k, e.N, 2) or tuple(out.shape) != (batch, 2, e.k, e.N, 2):
+        raise ValueError("cupoly_amd: relinearize operand shapes")
+    _check(lib().cupoly_relinearize(ctx.handle, key.handle, _ptr(out), _ptr(y), batch))
+
+
+def hom_mul_relin_batch(ctx, key, out, ct1, ct2):
+    """Full BFV multiplication: tensor product, HPS rescale, relinearization;
+    ct1, ct2 [batch, 2, k, N, 2] -> out [batch, 2, k, N, 2]."""
+    e = ctx.engine
+    batch = ct1.shape[0]
+    for t in (ct1, ct2, out):
+        if tuple(t.shape) != (batch, 2, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: hom_mul_relin_batch operand shape %s" % (tuple(t.shape),))
+    _check(lib().cupoly_hom_mul_relin_batch(ctx.handle, key.handle, _ptr(out), _ptr(ct1),
+                                            _ptr(ct2), batch))
+
+
+def hom_mul_relin_gather(ctx, key, outs, ct1s, ct2s):
+    """hom_mul_relin_batch over lists of per-request tensors [2, k, N, 2]."""
+    e = ctx.engine
+    b = len(outs)
+    if not (len(ct1s) == len(ct2s) == b):
+        raise ValueError("cupoly_amd: hom_mul_relin_gather list lengths differ")
+    for t in list(outs) + list(ct1s) + list(ct2s):
+        if tuple(t.shape) != (2, e.k, e.N, 2):
+            raise ValueError("cupoly_amd: hom_mul_relin_gather operand shape %s" % (tuple(t.shape),))
+    arr = ctypes.c_void_p * max(b, 1)
+    po = arr(*[_ptr(t).value for t in outs])
+    p1 = arr(*[_ptr(t).value for t in ct1s])
+    p2 = arr(*[_ptr(t).value for t in ct2s])
+    _check(lib().cupoly_hom_mul_relin_gather(ctx.handle, key.handle, po, p1, p2, b))
+
+
+def hom_mul_relin_workspace_bytes(engine, batch):
+    out = ctypes.c_uint64()
+    _check(lib().cupoly_hom_mul_relin_workspace_bytes(engine.handle, int(batch), ctypes.byref(out)))
+    return out.value
 
 
 NOALLOC_SCRATCH_BYTES = 128 << 20

@@ -512,20 +512,26 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     // b first: out may be b itself
     cp_gi *tb = nullptr;
     CP_HIP(hipMallocAsync((void **)&tb, bytes, c->stream));
-    CP_HIP(hipMemcpyAsync(tb, b, bytes, hipMemcpyDeviceToDevice, c->stream));
-    if (out != a) CP_HIP(hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream));
-    const int gcols = nunits * SP::WG_PER_POLY;
-    const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
-    hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, out,
-                       nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
-    hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, tb,
-                       nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
-    hipLaunchKernelGGL((k_polymul_blocks<LOGN>), dim3(gblks), dim3(1024), lds, c->stream, out,
-                       tb, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv);
-    hipLaunchKernelGGL((k_dgt_cols<LOGN, false>), dim3(gcols), dim3(1024), lds, c->stream, out,
-                       nunits, nres, 0, e->d_primes, e->d_tw_inv, e->d_twist_inv);
-    CP_LAUNCH_CHECK();
-    CP_HIP(hipFreeAsync(tb, c->stream));
+    // from here on every exit releases tb
+    hipError_t err = hipMemcpyAsync(tb, b, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (err == hipSuccess && out != a)
+      err = hipMemcpyAsync(out, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (err == hipSuccess) {
+      const int gcols = nunits * SP::WG_PER_POLY;
+      const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
+      hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, out,
+                         nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+      hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, tb,
+                         nunits, nres, 0, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+      hipLaunchKernelGGL((k_polymul_blocks<LOGN>), dim3(gblks), dim3(1024), lds, c->stream, out,
+                         tb, nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv);
+      hipLaunchKernelGGL((k_dgt_cols<LOGN, false>), dim3(gcols), dim3(1024), lds, c->stream, out,
+                         nunits, nres, 0, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+      err = hipGetLastError();
+    }
+    const hipError_t ferr = hipFreeAsync(tb, c->stream);
+    if (err != hipSuccess) return cp_check_hip(err, "polymul (split transforms)");
+    CP_HIP(ferr);
   }
   return CUPOLY_OK;
 }

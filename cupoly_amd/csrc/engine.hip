@@ -265,7 +265,8 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
       (rc = up((void **)&e->d_twist_inv, tsi.data(), tsi.size() * 8)) ||
       (rc = up((void **)&e->d_hps, &e->h_hps, sizeof(cp_hps))) ||
       (rc = up((void **)&e->d_crt, &e->h_crt, sizeof(cp_crt))) || (rc = cp_dgt_setup(e)) ||
-      (rc = cp_homomul_setup(e)) || (rc = cp_encrypt_setup(e))) {
+      (rc = cp_homomul_setup(e)) || (rc = cp_encrypt_setup(e)) ||
+      (rc = cp_relin_setup(e))) {
     std::string msg = g_last_error;
     cupoly_engine_destroy(e);
     return cp_fail(rc, msg);

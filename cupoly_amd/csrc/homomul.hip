@@ -862,6 +862,29 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
   return wide ? scale_to_q_wide(c, out, Y, X, batch) : scale_to_q(c, out, Y, batch);
 }
 
+// The full multiplication (relin.hip relinearizes): the rescaled tensor
+// T = [batch][3][k][n/2] words, then the scratch shared by the two steps.
+static size_t hom_mul_relin_ws_bytes(const cupoly_engine *e, int batch) {
+  const size_t t = (size_t)batch * 3 * e->k * e->ng * sizeof(cp_gi);
+  const size_t h = hom_mul_ws_bytes(e, batch), r = cp_relin_ws_bytes(e, batch);
+  return t + (h > r ? h : r);
+}
+
+static int hom_mul_relin_alloc_run(cupoly_ctx *c, const cupoly_relin_key *key,
+                                   cupoly_gi *const *d_out_table, cupoly_gi *out_base, cp_cts ct1,
+                                   cp_cts ct2, int batch) {
+  if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
+  void *ws;
+  int rc = cp_workspace(c, hom_mul_relin_ws_bytes(c->e, batch), &ws);
+  if (rc) return rc;
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_gi *T = (cp_gi *)ws;
+  void *shared = T + (size_t)batch * 3 * k * ng;
+  cp_cts t{T, 3 * k * ng, nullptr};
+  if ((rc = hom_mul_run(c, t, ct1, ct2, batch, shared))) return rc;
+  return cp_hom_mul_relin_run(c, key, d_out_table, out_base, T, batch, shared);
+}
+
 static int hom_mul_alloc_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   void *ws;
@@ -881,18 +904,15 @@ extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_
   return hom_mul_alloc_run(c, o, a, b, batch);
 }
 
-extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
-                                     const cupoly_gi *const *ct1, const cupoly_gi *const *ct2,
-                                     int batch) {
-  if (c && batch == 0) return CUPOLY_OK;  // empty batch: tables may be null
-  if (!c || !out || !ct1 || !ct2 || batch < 0)
-    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
+// Stages the gathered form's pointer tables (out, ct1, ct2) on the context's
+// stream, from a pinned slot that is not rewritten until its previous copy
+// has finished; *d receives the device table.
+static int stage_ptr_tables(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *const *ct1,
+                            const cupoly_gi *const *ct2, int batch, cp_gi ***d) {
   // the limit hom_mul_alloc_run enforces, checked before the tables grow
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   for (int i = 0; i < batch; i++)
     if (!out[i] || !ct1[i] || !ct2[i]) return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: null entry");
-  // pointer tables to the device on the context's stream, from a pinned
-  // staging slot that is not rewritten until its previous copy has finished
   const size_t need = (size_t)batch * 3 * sizeof(void *);
   int rc = grow_ptr_tables(c, need);
   if (rc) return rc;
@@ -909,19 +929,67 @@ extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
   CP_HIP(hipMemcpyAsync(c->d_ptrs, h, need, hipMemcpyHostToDevice, c->stream));
   CP_HIP(hipEventRecord(c->ring_ev[slot], c->stream));
   c->ring_used[slot] = true;
-  cp_gi **d = (cp_gi **)c->d_ptrs;
+  *d = (cp_gi **)c->d_ptrs;
+  return CUPOLY_OK;
+}
+
+extern "C" int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out,
+                                     const cupoly_gi *const *ct1, const cupoly_gi *const *ct2,
+                                     int batch) {
+  if (c && batch == 0) return CUPOLY_OK;  // empty batch: tables may be null
+  if (!c || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_gather: bad argument");
+  cp_gi **d;
+  int rc = stage_ptr_tables(c, out, ct1, ct2, batch, &d);
+  if (rc) return rc;
   cp_cts o{nullptr, 0, d}, a{nullptr, 0, d + batch}, b{nullptr, 0, d + 2 * batch};
   return hom_mul_alloc_run(c, o, a, b, batch);
+}
+
+extern "C" int cupoly_hom_mul_relin_batch(cupoly_ctx *c, const cupoly_relin_key *key,
+                                          cupoly_gi *out, const cupoly_gi *ct1,
+                                          const cupoly_gi *ct2, int batch) {
+  if (c && key && batch == 0) return CUPOLY_OK;
+  if (!c || !key || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_relin_batch: bad argument");
+  if (cp_relin_key_engine(key) != c->e)
+    return cp_fail(CUPOLY_ERR_STATE, "hom_mul_relin_batch: key of another engine");
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_cts a{(cp_gi *)ct1, 2 * k * ng, nullptr}, b{(cp_gi *)ct2, 2 * k * ng, nullptr};
+  return hom_mul_relin_alloc_run(c, key, nullptr, out, a, b, batch);
+}
+
+extern "C" int cupoly_hom_mul_relin_gather(cupoly_ctx *c, const cupoly_relin_key *key,
+                                           cupoly_gi *const *out, const cupoly_gi *const *ct1,
+                                           const cupoly_gi *const *ct2, int batch) {
+  if (c && key && batch == 0) return CUPOLY_OK;
+  if (!c || !key || !out || !ct1 || !ct2 || batch < 0)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_relin_gather: bad argument");
+  if (cp_relin_key_engine(key) != c->e)
+    return cp_fail(CUPOLY_ERR_STATE, "hom_mul_relin_gather: key of another engine");
+  cp_gi **d;
+  int rc = stage_ptr_tables(c, out, ct1, ct2, batch, &d);
+  if (rc) return rc;
+  cp_cts a{nullptr, 0, d + batch}, b{nullptr, 0, d + 2 * batch};
+  return hom_mul_relin_alloc_run(c, key, (cupoly_gi *const *)d, nullptr, a, b, batch);
 }
 
 extern "C" int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch) {
   if (!c || batch < 0 || batch > 65535 / 4)
     return cp_fail(CUPOLY_ERR_ARG, "hom_mul_reserve: bad argument");
   if (batch == 0) return CUPOLY_OK;
-  void *ws;
-  int rc = cp_workspace(c, hom_mul_ws_bytes(c->e, batch), &ws);
+  void *ws;  // room for the relinearizing form, the larger of the two
+  int rc = cp_workspace(c, hom_mul_relin_ws_bytes(c->e, batch), &ws);
   if (rc) return rc;
   return grow_ptr_tables(c, (size_t)batch * 3 * sizeof(void *));
+}
+
+extern "C" int cupoly_hom_mul_relin_workspace_bytes(const cupoly_engine *e, int batch,
+                                                    uint64_t *bytes) {
+  if (!e || !bytes || batch < 0 || batch > 65535 / 4)
+    return cp_fail(CUPOLY_ERR_ARG, "hom_mul_relin_workspace_bytes: bad argument");
+  *bytes = hom_mul_relin_ws_bytes(e, batch);
+  return CUPOLY_OK;
 }
 
 extern "C" int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain) {

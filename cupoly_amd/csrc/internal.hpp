@@ -112,4 +112,12 @@ int cp_dgt_setup(cupoly_engine *e);
 int cp_homomul_setup(cupoly_engine *e);
 int cp_encrypt_setup(cupoly_engine *e);
 int cp_workspace(cupoly_ctx *c, size_t bytes, void **out);
+int cp_relin_setup(cupoly_engine *e);
+// relinearization (relin.hip): scratch beyond its input, and the run over
+// T = [batch][3][k][n/2] coefficient-domain words into out (a device table of
+// `batch` pointers, or out_base + i * 2 k n/2 when the table is null)
+size_t cp_relin_ws_bytes(const cupoly_engine *e, int batch);
+const cupoly_engine *cp_relin_key_engine(const cupoly_relin_key *key);
+int cp_hom_mul_relin_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out_table,
+                         cupoly_gi *out_base, const cp_gi *T, int batch, void *ws);
 int cp_pointwise_mul_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nres);

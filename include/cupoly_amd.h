@@ -27,6 +27,7 @@ typedef struct { uint64_t re, imag; } cupoly_gi; /* GaussianInteger, settings.h:
 typedef struct cupoly_engine cupoly_engine;     /* CUDAEngine + DGTEngine state */
 typedef struct cupoly_ctx cupoly_ctx;           /* Context (stream + scratch)   */
 typedef struct cupoly_uneven cupoly_uneven;     /* plan for an uneven batch      */
+typedef struct cupoly_relin_key cupoly_relin_key; /* resident relinearization key */
 
 /* Error codes (the reference asserts / throws std::runtime_error / exit(1)). */
 enum {
@@ -218,8 +219,9 @@ int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1, co
 int cupoly_hom_mul_gather(cupoly_ctx *c, cupoly_gi *const *out, const cupoly_gi *const *ct1,
                           const cupoly_gi *const *ct2, int batch);
 /* Grows the context's hom-mul workspace and gathered-form pointer tables to
- * hold `batch` pairs, so later hom_mul_batch / hom_mul_gather calls of up to
- * that many pairs on this context allocate nothing (no reference counterpart:
+ * hold `batch` pairs, so later hom_mul_batch / hom_mul_gather calls (and
+ * their relinearizing forms) of up to that many pairs on this context
+ * allocate nothing (no reference counterpart:
  * the reference allocates its Context scratch once, context.cpp:19-67; a
  * serving loop calls this before it starts timing requests). */
 int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
@@ -249,6 +251,41 @@ int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain);
  * ring shape, context.cpp:19-67): 16 * n/2 * batch * (4 kl + 3 (k + kl)) for
  * n = 2^11 .. 2^14, 16 * n/2 * batch * 7 (k + kl) otherwise. */
 int cupoly_hom_mul_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes);
+
+/* ---- relinearization (no single reference entry point: the reference
+ * supplies its pieces, poly_xi_bfv / poly_rho_bfv, polynomial.cu:610-693 ->
+ * cudaengine.cu:992-1062, and poly_dot, polynomial.cu:1439-1451, and leaves
+ * key switching to the scheme built on it) ----------------------------------
+ * Key for secret s (base Q, coefficient domain, [k][n/2] GIs holding the
+ * residues of small integers): rlk_i = (b_i, a_i), i < k, a_i uniform mod Q
+ * (Philox stream stream0 + 2i under seed), e_i discrete Gaussian (stream
+ * stream0 + 2i + 1), b_i = -(a_i s + e_i) + s^2 (Q/q_i) mod Q; kept on the
+ * device in the DGT domain, [k][2][k][n/2] GIs ([i][0] = b_i, [i][1] = a_i),
+ * until cupoly_relin_key_destroy.  Creation synchronises the stream. */
+int cupoly_relin_key_create(cupoly_ctx *c, const cupoly_gi *s, uint64_t seed, uint64_t stream0,
+                            cupoly_relin_key **out);
+int cupoly_relin_key_destroy(cupoly_relin_key *key);
+/* Device address of the key's [k][2][k][n/2] GIs (read-only). */
+int cupoly_relin_key_data(const cupoly_relin_key *key, const cupoly_gi **data);
+/* Copies the key's words to dst (device, [k][2][k][n/2] GIs) on c's stream. */
+int cupoly_relin_key_copy(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *dst);
+/* in[i] = (y0, y1, y2), [batch][3][k][n/2] GIs, base Q, coefficient domain ->
+ * out[i] = (y0 + sum_j D_j b_j, y1 + sum_j D_j a_j) mod Q, [batch][2][k][n/2],
+ * with the RNS digits D_j = [y2_j (Q/q_j)^-1]_{q_j} (the reference's xi)
+ * written to every residue; out must not be in.  Then
+ * out0 + out1 s = y0 + y1 s + y2 s^2 - sum_j D_j e_j (mod Q). */
+int cupoly_relinearize(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *out,
+                       const cupoly_gi *in, int batch);
+/* The full BFV multiplication: cupoly_hom_mul_batch followed by
+ * cupoly_relinearize; out[i] = [2][k][n/2].  Workspace:
+ * cupoly_hom_mul_relin_workspace_bytes (cupoly_hom_mul_reserve reserves it). */
+int cupoly_hom_mul_relin_batch(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *out,
+                               const cupoly_gi *ct1, const cupoly_gi *ct2, int batch);
+/* Gathered form (serving): out[i] -> [2][k][n/2], ct1[i], ct2[i] -> [2][k][n/2]. */
+int cupoly_hom_mul_relin_gather(cupoly_ctx *c, const cupoly_relin_key *key,
+                                cupoly_gi *const *out, const cupoly_gi *const *ct1,
+                                const cupoly_gi *const *ct2, int batch);
+int cupoly_hom_mul_relin_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes);
 
 /* Encryption-shaped step for a batch: out[i] = a[i] * s_i + e_i in the DGT
  * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian

@@ -379,3 +379,83 @@ def sample_muladd(P, a, seed, stream0):
         e = fold(small_to_residues(sample_small(P.n, 3, seed, stream0 + 2 * i + 1), P.q), P.Ng)
         out[i] = pointwise(P, a[i], dgt(P, s, 0, 0), dgt(P, e, 0, 0), base=0, op=3)
     return out
+
+
+def relin_key(P, s_vals, seed, stream0):
+    """Relinearization key for the secret with signed coefficients s_vals [n],
+    as the library defines it (BFV relinearization over the reference's
+    rho, polynomial.cu:610-693): a_i = DGT(uniform, stream stream0 + 2i),
+    e_i = DGT(Gaussian, stream0 + 2i + 1), b_i = -(a_i s + e_i) + rho(s^2)_i,
+    all in the DGT domain of base Q.  -> [k, 2, k, Ng, 2] ([i][0] = b_i)."""
+    n, k = P.n, P.k
+    sh = dgt(P, fold(small_to_residues(s_vals, P.q), P.Ng), 0, 0)
+    rho = rho_bfv(P, pointwise(P, sh, sh, op=2))
+    key = np.empty((k, 2, k, P.Ng, 2), dtype=np.uint64)
+    for i in range(k):
+        a = dgt(P, fold(sample_uniform(P.q, n, seed, stream0 + 2 * i), P.Ng), 0, 0)
+        e = dgt(P, fold(small_to_residues(sample_small(n, 3, seed, stream0 + 2 * i + 1), P.q),
+                        P.Ng), 0, 0)
+        key[i, 1] = a
+        key[i, 0] = pointwise(P, rho[i], pointwise(P, a, sh, e, op=3), op=1)
+    return key
+
+
+def relin_digits(P, y2):
+    """RNS digits of y2 (GI [k, Ng, 2], coefficient domain): digit i is the
+    integer [y2_i (Q/q_i)^-1]_{q_i} (residue i of the reference's xi,
+    cudaengine.cu:992-1035), written to every residue j as its value mod q_j.
+    -> [k (digit), k (residue), Ng, 2]."""
+    k = P.k
+    y2 = u64(y2)
+    out = np.empty((k, k, P.Ng, 2), dtype=np.uint64)
+    for i in range(k):
+        d = np.zeros((P.Ng, 2), dtype=np.uint64)
+        lib().or_scale_residue(_p(d), _p(np.ascontiguousarray(y2[i])), P.Ng, P.q[i], P.invQqi[i])
+        for j, qj in enumerate(P.q):
+            out[i, j] = d % np.uint64(qj)
+    return out
+
+
+def relinearize(P, y, key):
+    """y = (y0, y1, y2) GI [3, k, Ng, 2] coefficient domain, key [k, 2, k, Ng, 2]
+    (DGT domain) -> (y0 + sum_i D_i b_i, y1 + sum_i D_i a_i) [2, k, Ng, 2]."""
+    y = u64(y)
+    D = relin_digits(P, y[2])
+    out = []
+    for m in range(2):
+        acc = np.zeros((P.k, P.Ng, 2), dtype=np.uint64)
+        for i in range(P.k):
+            acc = pointwise(P, dgt(P, D[i], 0, 0), key[i, m], acc, op=3)
+        out.append(pointwise(P, y[m], dgt(P, acc, 0, 1), op=0))
+    return np.stack(out)
+
+
+def negacyclic_mod_t(m1, m2, t):
+    """m1 * m2 mod (X^n + 1, t) for integer vectors in [0, t), t < 2^17."""
+    n = len(m1)
+    full = np.convolve(np.asarray(m1, dtype=np.int64), np.asarray(m2, dtype=np.int64))
+    res = full[:n].copy()
+    res[:n - 1] -= full[n:]
+    return np.mod(res, t)
+
+
+def bfv_encrypt(P, m, s_vals, seed, stream):
+    """Textbook BFV encryption of m (integers in [0, t), length n) under the
+    secret with signed coefficients s_vals: c1 = a uniform, c0 = Delta m -
+    a s + e, Delta = floor(Q / t), e discrete Gaussian.  -> GI [2, k, Ng, 2]."""
+    a = fold(sample_uniform(P.q, P.n, seed, stream), P.Ng)
+    e = fold(small_to_residues(sample_small(P.n, 3, seed, stream + 1), P.q), P.Ng)
+    s = fold(small_to_residues(s_vals, P.q), P.Ng)
+    delta = P.Q // P.t
+    dm = fold(np.array([[(int(x) * (delta % qj)) % qj for x in m] for qj in P.q],
+                       dtype=np.uint64), P.Ng)
+    a_s = polymul_batch(P, a[None], s[None])[0]
+    c0 = pointwise(P, pointwise(P, dm, e, op=0), a_s, op=1)
+    return np.stack([c0, a])
+
+
+def bfv_decrypt(P, ct, s_vals):
+    """round(t/Q [c0 + c1 s]_Q) mod t for a 2-polynomial ciphertext -> int64 [n]."""
+    s = fold(small_to_residues(s_vals, P.q), P.Ng)
+    x = pointwise(P, u64(ct[0]), polymul_batch(P, u64(ct[1])[None], s[None])[0], op=0)
+    return unfold(simple_scaling(P, x)).astype(np.int64)

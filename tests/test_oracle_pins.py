@@ -632,3 +632,44 @@ def test_hom_mul_composition_matches_exact_rescaled_tensor():
             for j, q in enumerate(P.q):
                 d = (int(flat[j, c]) - y) % q
                 assert d in (0, 1, q - 1), (m, c, j)
+
+
+def test_oracle_bfv_multiply_relinearize_decrypts():
+    """The oracle's relinearization key and relinearization, pinned by what they
+    are for: encrypt m1 and m2 (textbook BFV, ternary secret), multiply with the
+    oracle's hom_mul (tensor + HPS rescale), relinearize with the key, decrypt
+    with the secret alone: m1 * m2 mod (X^n + 1, t).  Also: every key pair
+    satisfies b_i + a_i s = rho(s^2)_i - e_i with e_i the Gaussian draw of its
+    counters, and the unrelinearized tensor decrypts with s and s^2."""
+    n, k, t = 1024, 3, 65537
+    P = O.Params(k, k + 1, n, t)
+    rng = np.random.default_rng(31)
+    s_vals = O.sample_small(n, 2, 17, 0)
+    m1, m2 = rng.integers(0, t, n), rng.integers(0, t, n)
+    ct1 = O.bfv_encrypt(P, m1, s_vals, 17, 10)
+    ct2 = O.bfv_encrypt(P, m2, s_vals, 17, 20)
+    assert np.array_equal(O.bfv_decrypt(P, ct1, s_vals), m1)
+    key = O.relin_key(P, s_vals, 17, 100)
+    sh = O.dgt(P, O.fold(O.small_to_residues(s_vals, P.q), P.Ng), 0, 0)
+    rho = O.rho_bfv(P, O.pointwise(P, sh, sh, op=2))
+    for i in range(k):
+        lhs = O.pointwise(P, key[i, 0], O.pointwise(P, key[i, 1], sh, op=2), op=0)
+        e = O.dgt(P, O.pointwise(P, rho[i], lhs, op=1), 0, 1)  # = e_i
+        want = O.small_to_residues(O.sample_small(n, 3, 17, 100 + 2 * i + 1), P.q)
+        assert np.array_equal(O.unfold(e), want), i
+    y = O.hom_mul(P, ct1, ct2)
+    want = O.negacyclic_mod_t(m1, m2, t)
+    assert np.array_equal(O.bfv_decrypt(P, O.relinearize(P, y, key), s_vals), want)
+    # planted mistake: digits present in their own residue only (the
+    # reference's xi as it is) satisfy sum_i rho(a)_i xi(b)_i = a b but do
+    # not relinearize: the key's noise terms need the digit in every residue
+    D = O.relin_digits(P, y[2])
+    wrong = []
+    for m in range(2):
+        acc = np.zeros((k, P.Ng, 2), dtype=np.uint64)
+        for i in range(k):
+            Di = np.zeros_like(D[i])
+            Di[i] = D[i, i]
+            acc = O.pointwise(P, O.dgt(P, Di, 0, 0), key[i, m], acc, op=3)
+        wrong.append(O.pointwise(P, y[m], O.dgt(P, acc, 0, 1), op=0))
+    assert not np.array_equal(O.bfv_decrypt(P, np.stack(wrong), s_vals), want)
