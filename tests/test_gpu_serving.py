@@ -1,5 +1,6 @@
 """Continuous batching: every request's result in the mixed, overlapped run is
-bit-identical to the same multiplication run alone (batch of one)."""
+bit-identical to the same multiplication (tensor, rescale, relinearization
+with the shape's resident key) run alone (batch of one)."""
 import pytest
 import torch
 
@@ -21,20 +22,28 @@ def test_serving_results_equal_alone(gpu, shape_args, probs, rate):
     assert stats["requests"] == 40
     assert stats["batches"] >= 2
     assert stats["p99_ms"] >= stats["p50_ms"] > 0
+    torch.cuda.synchronize()
     for r in reqs:
         alone = cp.empty(r.out.shape)
-        cp.hom_mul_batch(r.shape.ctx, alone[None], r.ct1[None], r.ct2[None])
-        r.shape.ctx.sync()
+        ctx = cp.Context(r.shape.engine)  # torch's current stream: ordered after the inputs
+        cp.hom_mul_relin_batch(ctx, r.shape.key, alone[None], r.ct1[None], r.ct2[None])
+        torch.cuda.synchronize()
+        ctx.destroy()
         assert torch.equal(alone, r.out), r.rid
 
 
 def test_admission_respects_hbm_budget(gpu):
     shapes = [serving.Shape(2048, 2)]
     reqs = serving.make_requests(shapes, [1.0], 30, 1e6, seed=6)
-    budget = 3 * shapes[0].footprint  # at most three requests admitted at once
-    stats = serving.Server(shapes, hbm_budget=budget).run(reqs)
+    sh = shapes[0]
+    reserved = len(sh.ctxs) * sh.workspace_bytes(3)
+    budget = reserved + 3 * sh.footprint  # at most three outputs admitted at once
+    srv = serving.Server(shapes, hbm_budget=budget, max_batch={sh: 3})
+    assert srv.reserved == reserved
+    stats = srv.run(reqs)
     assert stats["requests"] == 30
-    assert stats["peak_admitted_bytes"] <= budget
+    assert stats["reserved_workspace_bytes"] == reserved
+    assert reserved < stats["peak_admitted_bytes"] <= budget
 
 
 def test_hom_mul_reserve_keeps_results(gpu):
@@ -47,8 +56,11 @@ def test_hom_mul_reserve_keeps_results(gpu):
     b = cp.empty((3,) + sh.out_shape)
     ct1 = torch.stack([r.ct1 for r in reqs])
     ct2 = torch.stack([r.ct2 for r in reqs])
-    cp.hom_mul_batch(sh.ctx, a, ct1, ct2)
-    cp.hom_mul_batch(fresh, b, ct1, ct2)
+    # the stacks and zero fills ran on torch's current stream; the lane's
+    # stream and the fresh context's stream wait for them
+    torch.cuda.synchronize()
+    cp.hom_mul_relin_batch(sh.ctx, sh.key, a, ct1, ct2)
+    cp.hom_mul_relin_batch(fresh, sh.key, b, ct1, ct2)
     sh.ctx.sync()
     fresh.sync()
     assert torch.equal(a, b)
