@@ -271,6 +271,21 @@ def main():
     torch.cuda.synchronize()
     plain_rate = PB * reps / (time.perf_counter() - t1)
     assert torch.equal(pout, out[:PB]), "plain and batched results differ"
+    # latency of one product alone: plain (the reference's 7 launches: two
+    # per transform, one for the slot-wise product) and fused (one launch)
+    one_a, one_b, one_o = a[:1].clone(), b[:1].clone(), cp.empty((1, k, ng, 2))
+    lat_reps = 20
+    torch.cuda.synchronize()
+    t6 = time.perf_counter()
+    for _ in range(lat_reps):
+        cp.polymul_plain(ctx, one_o, one_a, one_b)  # a, b stay transformed: same work
+        torch.cuda.synchronize()
+    plain_one_ms = (time.perf_counter() - t6) * 1e3 / lat_reps
+    t7 = time.perf_counter()
+    for _ in range(lat_reps):
+        cp.polymul_batch(ctx, one_o, a[:1], b[:1])
+        torch.cuda.synchronize()
+    fused_one_ms = (time.perf_counter() - t7) * 1e3 / lat_reps
     print("bench: plain done %.0f poly-mul/s" % plain_rate, file=sys.stderr, flush=True)
 
     # BFV multiplication core (tensor product + HPS Q->B, t/Q, B->Q), t = 65537
@@ -436,7 +451,14 @@ def main():
                    "n": n, "k": k, "batch": B, "primes": "63-bit bucket q_0..q_%d" % (k - 1),
                    "parallelism": "single GPU"},
         "plain": {"value": plain_rate, "unit": "poly-muls/s", "batch": PB,
-                  "launches_per_product": 4},
+                  "launches_per_product": 7,
+                  "form": "one product at a time in the reference's order: forward a, "
+                          "forward b (column and block kernels each, every residue "
+                          "polynomial spread over 2^(log2(n/2)-9) 64-thread workgroups), "
+                          "slot-wise product, inverse (two kernels)",
+                  "workgroups_per_transform_kernel": k * max(1, (ng >> 9)),
+                  "single_product_ms": plain_one_ms,
+                  "fused_single_product_ms": fused_one_ms},
         "speedup_vs_plain": value / plain_rate,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

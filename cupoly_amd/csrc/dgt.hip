@@ -92,28 +92,37 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS)
 }
 
 // -------------------------------------------------------------------------
-// Two-kernel transform for N = 2^LOGN > 8192: S = LOGN - 7 column stages
-// handled as 2^LOGC consecutive columns per workgroup; the remaining 7 stages
-// on contiguous blocks of 128 slots, 2^(13-7) blocks per workgroup.
+// Two-kernel transform: S = LOGN - 7 column stages handled as 2^LOGC
+// consecutive columns per workgroup, then the remaining 7 stages on
+// contiguous blocks of 128 slots, 2^(LOGM-7) blocks per workgroup; both
+// kernels work on a local array of 2^LOGM slots (2^(LOGM-3) threads).
+// LOGM = 13 (1024 threads) for N = 16384, 32768; LOGM = 9 (64 threads)
+// spreads one residue polynomial of N = 1024 .. 32768 slots over
+// 2^(LOGN-9) workgroups per kernel, for calls with too few residue
+// polynomials to fill the GPU one workgroup each.
 // -------------------------------------------------------------------------
-template <int LOGN>
+template <int LOGN, int LOGM_ = 13>
 struct Split {
   static constexpr int LOGB = 7;                 // block size 128
   static constexpr int S = LOGN - LOGB;          // column stages; column length 2^S
   static constexpr int LOGNS = LOGN - S;         // column stride = 2^LOGB
-  static constexpr int LOGC = 13 - S;            // columns per workgroup
-  static constexpr int LOGM = 13;                // local array 8192 slots
+  static constexpr int LOGM = LOGM_;             // local array 2^LOGM slots
+  static constexpr int LOGC = LOGM - S;          // columns per workgroup
   static constexpr int WG_PER_POLY = (1 << LOGNS) >> LOGC;
   static constexpr int BLOCKS_PER_WG = (1 << LOGM) >> LOGB;
+  static constexpr int THREADS = (1 << LOGM) / EPT;
+  static_assert(LOGC >= 0 && LOGC <= LOGNS, "column split does not fit the local array");
 };
+// log2 of the local array of the wide split (64-thread workgroups)
+constexpr int kWideLogM = 9;
 
 // Column stages [0, S): forward (twist first) or inverse (untwist last).
-template <int LOGN, bool FWD>
-__global__ void __launch_bounds__(1024)
+template <int LOGN, bool FWD, int LOGM_ = 13>
+__global__ void __launch_bounds__((1 << LOGM_) / EPT)
     k_dgt_cols(cp_gi *__restrict__ data, int nunits, int nres, int roff,
                const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all,
                const uint64_t *__restrict__ twist_all) {
-  using SP = Split<LOGN>;
+  using SP = Split<LOGN, LOGM_>;
   constexpr int N = 1 << LOGN, LOGM = SP::LOGM;
   extern __shared__ cp_gi lds[];
   const int t = threadIdx.x;
@@ -159,11 +168,11 @@ __global__ void __launch_bounds__(1024)
 }
 
 // Block stages [S, LOGN): each workgroup row is one 128-slot block.
-template <int LOGN, bool FWD>
-__global__ void __launch_bounds__(1024)
+template <int LOGN, bool FWD, int LOGM_ = 13>
+__global__ void __launch_bounds__((1 << LOGM_) / EPT)
     k_dgt_blocks(cp_gi *__restrict__ data, int nunits, int nres, int roff,
                  const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all) {
-  using SP = Split<LOGN>;
+  using SP = Split<LOGN, LOGM_>;
   constexpr int N = 1 << LOGN, LOGB = SP::LOGB, B = 1 << LOGB, TB = B / EPT;
   extern __shared__ cp_gi lds_all[];
   const int row = threadIdx.x / TB, t = threadIdx.x % TB;
@@ -396,11 +405,48 @@ int cp_dgt_setup(cupoly_engine *e) {
   }
 }
 
+template <int LOGN, int LOGM>
+static void launch_split(cupoly_ctx *c, cp_gi *data, int roff, int nres, int nunits, int dir) {
+  using SP = Split<LOGN, LOGM>;
+  cupoly_engine *e = c->e;
+  const size_t lds_c = (size_t)lds_row<LOGM>() * sizeof(cp_gi);
+  const size_t lds_b = (size_t)SP::BLOCKS_PER_WG * lds_row<SP::LOGB>() * sizeof(cp_gi);
+  const int gcols = nunits * SP::WG_PER_POLY;
+  const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
+  const dim3 th(SP::THREADS);
+  if (dir == CUPOLY_FORWARD) {
+    hipLaunchKernelGGL((k_dgt_cols<LOGN, true, LOGM>), dim3(gcols), th, lds_c, c->stream, data,
+                       nunits, nres, roff, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+    hipLaunchKernelGGL((k_dgt_blocks<LOGN, true, LOGM>), dim3(gblks), th, lds_b, c->stream, data,
+                       nunits, nres, roff, e->d_primes, e->d_tw_fwd);
+  } else {
+    hipLaunchKernelGGL((k_dgt_blocks<LOGN, false, LOGM>), dim3(gblks), th, lds_b, c->stream,
+                       data, nunits, nres, roff, e->d_primes, e->d_tw_inv);
+    hipLaunchKernelGGL((k_dgt_cols<LOGN, false, LOGM>), dim3(gcols), th, lds_c, c->stream, data,
+                       nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+  }
+}
+
+// One transform launch form per call: N <= 8192 slots in one workgroup per
+// residue polynomial (k_dgt_single), N = 16384, 32768 in the 1024-thread
+// split; when that would give fewer workgroups than CUs and N >= 1024, the
+// 64-thread split, which spreads each residue polynomial over 2^(LOGN-9)
+// workgroups per kernel (the reference's two-pass form, dgt.cu:125-160,
+// whose passes run on Na * nres blocks).
 template <int LOGN>
 static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npoly, int dir) {
   cupoly_engine *e = c->e;
   const int nunits = nres * npoly;
   if (nunits <= 0) return CUPOLY_OK;
+  if constexpr (LOGN >= 10) {
+    int wgs = nunits;
+    if constexpr (LOGN > 13) wgs *= Split<LOGN>::WG_PER_POLY;
+    if (wgs < e->ncu) {
+      launch_split<LOGN, kWideLogM>(c, data, roff, nres, nunits, dir);
+      CP_LAUNCH_CHECK();
+      return CUPOLY_OK;
+    }
+  }
   if constexpr (LOGN <= 13) {
     constexpr int R = rows_for<LOGN>(), T = (1 << LOGN) / EPT;
     const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
@@ -412,21 +458,7 @@ static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npol
       hipLaunchKernelGGL((k_dgt_single<LOGN, R, false>), dim3(grid), dim3(T * R), lds, c->stream,
                          data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
   } else {
-    using SP = Split<LOGN>;
-    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
-    const int gcols = nunits * SP::WG_PER_POLY;
-    const int gblks = (nunits << (LOGN - SP::LOGB)) / SP::BLOCKS_PER_WG;
-    if (dir == CUPOLY_FORWARD) {
-      hipLaunchKernelGGL((k_dgt_cols<LOGN, true>), dim3(gcols), dim3(1024), lds, c->stream, data,
-                         nunits, nres, roff, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
-      hipLaunchKernelGGL((k_dgt_blocks<LOGN, true>), dim3(gblks), dim3(1024), lds, c->stream,
-                         data, nunits, nres, roff, e->d_primes, e->d_tw_fwd);
-    } else {
-      hipLaunchKernelGGL((k_dgt_blocks<LOGN, false>), dim3(gblks), dim3(1024), lds, c->stream,
-                         data, nunits, nres, roff, e->d_primes, e->d_tw_inv);
-      hipLaunchKernelGGL((k_dgt_cols<LOGN, false>), dim3(gcols), dim3(1024), lds, c->stream,
-                         data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
-    }
+    launch_split<LOGN, 13>(c, data, roff, nres, nunits, dir);
   }
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;

@@ -795,15 +795,36 @@ static int grow_ptr_tables(cupoly_ctx *c, size_t need) {
 
 // (k + kl)-residue scratch of the pipeline: 4 extended inputs and 3 products per pair.
 // n = 2^11 .. 2^14: one workgroup runs every transform of a (pair, residue)
+// once the batch has a (pair, residue) unit for every CU.  Smaller batches
+// take the composed transforms, which spread each residue polynomial over
+// several workgroups: seven transforms in series in one workgroup would set
+// the latency of a batch that leaves most CUs idle.
 static bool hom_mul_fti_sizes(const cupoly_engine *e) { return e->logng >= 10 && e->logng <= 13; }
+static int hom_mul_fti_min_batch(const cupoly_engine *e) {
+  const int nres = e->k + e->kl;
+  return (e->ncu + nres - 1) / nres;
+}
+static bool hom_mul_uses_fti(const cupoly_engine *e, int batch) {
+  return hom_mul_fti_sizes(e) && batch >= hom_mul_fti_min_batch(e);
+}
 
-// Scratch of the pipeline per pair: the extended inputs X (4 polynomials of
-// the B rows only when the fused transforms read the Q rows from the
-// ciphertexts, of all k + kl rows otherwise) and the 3 products Y (k + kl
-// rows), N Gaussian slots each.
-static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
-  const size_t xrows = hom_mul_fti_sizes(e) ? e->kl : e->k + e->kl;
+// Scratch of the pipeline for `batch` pairs in one launch form: the extended
+// inputs X (4 polynomials of the B rows only when the fused transforms read
+// the Q rows from the ciphertexts, of all k + kl rows otherwise) and the 3
+// products Y (k + kl rows), N Gaussian slots each.
+static size_t hom_mul_ws_form(const cupoly_engine *e, int batch, bool fti_form) {
+  const size_t xrows = fti_form ? e->kl : e->k + e->kl;
   return (size_t)batch * (4 * xrows + 3 * (size_t)(e->k + e->kl)) * e->ng * sizeof(cp_gi);
+}
+
+// Scratch that covers every batch of up to `batch` pairs (the two forms need
+// different amounts per pair, so the figure is the larger over both ranges).
+static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
+  if (!hom_mul_fti_sizes(e)) return hom_mul_ws_form(e, batch, false);
+  const int b0 = hom_mul_fti_min_batch(e);
+  const size_t f = batch >= b0 ? hom_mul_ws_form(e, batch, true) : 0;
+  const size_t g = hom_mul_ws_form(e, batch < b0 ? batch : b0 - 1, false);
+  return f > g ? f : g;
 }
 
 // The pipeline on the context's stream with scratch `ws` of at least
@@ -811,7 +832,7 @@ static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
 static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch, void *ws) {
   cupoly_engine *e = c->e;
   const int k = e->k, kl = e->kl, nres = k + kl, ng = e->ng;
-  const bool fti_sizes = hom_mul_fti_sizes(e);
+  const bool fti_sizes = hom_mul_uses_fti(e, batch);
   const int xrows = fti_sizes ? kl : nres;
   const size_t xs = (size_t)batch * 4 * xrows * ng;
   cp_gi *X = (cp_gi *)ws, *Y = X + xs;

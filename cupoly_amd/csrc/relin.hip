@@ -253,15 +253,30 @@ int cp_relin_setup(cupoly_engine *e) {
   }
 }
 
+// The fused form once the batch has a (pair, residue) unit for every CU;
+// smaller batches take the composed form, whose transforms spread each
+// residue polynomial over several workgroups (k transforms in series in one
+// workgroup would set the latency of a batch that leaves most CUs idle).
 static bool relin_fused_sizes(const cupoly_engine *e) { return e->logng >= 10 && e->logng <= 13; }
-
-// Scratch of relin_run beyond its input, in residue polynomials per pair.
-static size_t relin_polys(const cupoly_engine *e) {
-  return relin_fused_sizes(e) ? (size_t)e->k : (size_t)e->k * e->k + 2 * (size_t)e->k;
+static int relin_fused_min_batch(const cupoly_engine *e) { return (e->ncu + e->k - 1) / e->k; }
+static bool relin_uses_fused(const cupoly_engine *e, int batch) {
+  return relin_fused_sizes(e) && batch >= relin_fused_min_batch(e);
 }
 
+// Scratch of relin_run beyond its input for `batch` pairs in one form: the
+// parked a-sums (k residue polynomials per pair) or the digits and both sums.
+static size_t relin_ws_form(const cupoly_engine *e, int batch, bool fused) {
+  const size_t polys = fused ? (size_t)e->k : (size_t)e->k * e->k + 2 * (size_t)e->k;
+  return (size_t)batch * polys * e->ng * sizeof(cp_gi);
+}
+
+// Scratch that covers every batch of up to `batch` pairs.
 size_t cp_relin_ws_bytes(const cupoly_engine *e, int batch) {
-  return (size_t)batch * relin_polys(e) * e->ng * sizeof(cp_gi);
+  if (!relin_fused_sizes(e)) return relin_ws_form(e, batch, false);
+  const int b0 = relin_fused_min_batch(e);
+  const size_t f = batch >= b0 ? relin_ws_form(e, batch, true) : 0;
+  const size_t g = relin_ws_form(e, batch < b0 ? batch : b0 - 1, false);
+  return f > g ? f : g;
 }
 
 // y = T [batch][3][k][ng] (coefficient domain, base Q) -> out[pair] = [2][k][ng];
@@ -271,7 +286,7 @@ static int relin_run(cupoly_ctx *c, const cupoly_relin_key *key, cp_rcts out, co
   cupoly_engine *e = c->e;
   const int k = e->k, ng = e->ng;
   const dim3 sg((ng + 255) / 256);
-  if (relin_fused_sizes(e)) {
+  if (relin_uses_fused(e, batch)) {
     cp_gi *park = (cp_gi *)ws;
     switch (e->logng) {
       case 10: return launch_relin_fti<10>(c, out, T, key->d, park, batch);

@@ -231,7 +231,7 @@ int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch);
  * scratch is a fixed device-resident area of CUPOLY_NOALLOC_SCRATCH_BYTES
  * (128 MiB) per GPU, held from the first engine created on that GPU until the
  * process ends; batches whose scratch (cupoly_hom_mul_workspace_bytes) exceeds
- * it fail with CUPOLY_ERR_ARG (at n = 2^14, k = 4: 21 pairs).  Every call on
+ * it fail with CUPOLY_ERR_ARG (at n = 2^14, k = 4: 16 pairs).  Every call on
  * one GPU shares that area: calls that overlap on one GPU (on different
  * streams, or from different threads) race and must be serialised by the
  * caller.  Same results as cupoly_hom_mul_batch. */
@@ -248,8 +248,10 @@ int cupoly_hom_mul_batch_noalloc(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi 
 int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain);
 /* Device bytes of the hom-mul workspace for `batch` pairs of e's shape (no
  * reference counterpart; the reference sizes its Context scratch from the
- * ring shape, context.cpp:19-67): 16 * n/2 * batch * (4 kl + 3 (k + kl)) for
- * n = 2^11 .. 2^14, 16 * n/2 * batch * 7 (k + kl) otherwise. */
+ * ring shape, context.cpp:19-67), enough for every batch of up to `batch`
+ * pairs: per pair 16 * n/2 * (4 kl + 3 (k + kl)) bytes in the fused form
+ * (n = 2^11 .. 2^14, batches of at least ceil(CUs / (k + kl)) pairs) and
+ * 16 * n/2 * 7 (k + kl) bytes otherwise. */
 int cupoly_hom_mul_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes);
 
 /* ---- relinearization (no single reference entry point: the reference

@@ -371,6 +371,30 @@ def test_hom_mul_scale_forms_agree(gpu, k, n, batch):
                           O.hom_mul(P, cp.to_host(ct1[i]), cp.to_host(ct2[i])))
 
 
+@pytest.mark.parametrize("k,n,batch", [(2, 2048, 52), (3, 4096, 37), (3, 8192, 37),
+                                         (4, 16384, 29)])
+def test_hom_mul_fused_and_composed_forms_agree(gpu, k, n, batch):
+    """Batches with a (pair, residue) unit per CU (256 CUs: ceil(256 / (k + kl))
+    pairs) run every transform of a unit in one workgroup; single pairs take
+    the composed transforms.  Both forms give the same words for every pair,
+    and two pairs equal the oracle composition."""
+    import torch
+    e, ctx = engine(k, k + 1, n, 65537)
+    P = O.Params(k, k + 1, n, 65537)
+    ct1 = rand_res(P, P.q, (batch, 2), 800 + n)
+    ct2 = rand_res(P, P.q, (batch, 2), 801 + n)
+    d1, d2 = cp.to_device(ct1), cp.to_device(ct2)
+    full = cp.empty((batch, 3, k, P.Ng, 2))
+    cp.hom_mul_batch(ctx, full, d1, d2)
+    one = cp.empty((1, 3, k, P.Ng, 2))
+    for i in range(batch):
+        cp.hom_mul_batch(ctx, one, d1[i:i + 1], d2[i:i + 1])
+        assert torch.equal(one[0], full[i]), i
+    h = cp.to_host(full)
+    for i in (0, batch - 1):
+        assert np.array_equal(h[i], O.hom_mul(P, ct1[i], ct2[i])), i
+
+
 def test_hom_mul_full_batch_properties(gpu):
     """n = 2^14, k = 4, 128 pairs (the benchmarked batch): the tensor product is
     symmetric, so swapping the ciphertexts leaves every pair's result unchanged;

@@ -52,9 +52,8 @@ def test_relin_key_matches_oracle(gpu, k, n):
 @pytest.mark.parametrize("k,n,batch", [(2, 2048, 3), (3, 4096, 2), (3, 8192, 3), (4, 16384, 2),
                                          (8, 32768, 1), (2, 64, 3)])
 def test_relinearize_matches_oracle(gpu, k, n, batch):
-    """n = 2^11 .. 2^14 take the one-workgroup-per-(pair, residue) kernel, the
-    others the composed form; both bit-exact against the oracle on uniform
-    (y0, y1, y2)."""
+    """Batches this small take the composed form (the fused one is checked
+    against it below); bit-exact against the oracle on uniform (y0, y1, y2)."""
     import torch
     e, ctx, P, s_vals, rk = setup(k, n)
     key = key_on_host(ctx, rk, P)
@@ -115,3 +114,27 @@ def test_bfv_multiply_relinearize_decrypts(gpu, k, n):
     cp.hom_mul_relin_batch(ctx, rk, out, cp.to_device(ct1[None]), cp.to_device(ct2[None]))
     got = O.bfv_decrypt(P, cp.to_host(out[0]), s_vals)
     assert np.array_equal(got, O.negacyclic_mod_t(m1, m2, t))
+
+
+@pytest.mark.parametrize("k,n,batch", [(2, 2048, 128), (3, 8192, 86), (4, 16384, 64)])
+def test_relinearize_fused_and_composed_forms_agree(gpu, k, n, batch):
+    """Batches with a (pair, residue) unit per CU (ceil(256 / k) pairs) take the
+    one-workgroup-per-unit kernel, single pairs the composed form: the same
+    words for every pair, and two pairs equal the oracle."""
+    import torch
+    e, ctx, P, s_vals, rk = setup(k, n)
+    rng = np.random.default_rng(11 * n + k)
+    y = np.empty((batch, 3, k, P.Ng, 2), dtype=np.uint64)
+    for j, qj in enumerate(P.q):
+        y[:, :, j] = rng.integers(0, qj, size=(batch, 3, P.Ng, 2), dtype=np.uint64)
+    dy = cp.to_device(y)
+    full = cp.empty((batch, 2, k, P.Ng, 2))
+    cp.relinearize(ctx, rk, full, dy)
+    one = cp.empty((1, 2, k, P.Ng, 2))
+    for i in range(batch):
+        cp.relinearize(ctx, rk, one, dy[i:i + 1])
+        assert torch.equal(one[0], full[i]), i
+    key = key_on_host(ctx, rk, P)
+    h = cp.to_host(full)
+    for i in (0, batch - 1):
+        assert np.array_equal(h[i], O.relinearize(P, y[i], key)), i

@@ -142,6 +142,7 @@ def rows_for(L):
 CASES = [(L, 0, L, rows_for(L)) for L in range(4, 14)]        # single, fused, a*s+e, tensor
 CASES += [(L, 0, L, 8192 >> L) for L in range(10, 14)]         # uneven tiles, 1024 threads
 CASES += [(13, 0, 7, 1), (13, 0, 8, 1), (7, 0, 7, 64)]         # column and block kernels
+CASES += [(9, 0, s, 1) for s in range(3, 9)] + [(7, 0, 7, 4)]  # 64-thread split, N = 2^10 .. 2^15
 
 
 @pytest.mark.parametrize("L,lsb,lse,rows", CASES)
