@@ -309,6 +309,14 @@ __device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
+// An opaque copy of a thread index: address arithmetic derived from it
+// cannot be hoisted above this point, nor shared between two pass sequences
+// and held in VGPRs across the transform in between (where it would spill).
+__device__ __forceinline__ int cp_opaque(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // ---- Exchange synchronisation ------------------------------------------------
 // An exchange between full radix-8 layouts at local stages ls0 and ls0 + 3
 // is wave-local when the earlier layout's unit stride 2^lst is at most 64
@@ -362,6 +370,7 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);  // partial <-> first full
+  t = cp_opaque(t);
   if constexpr (PP::R0 > 0) {
     dif_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
     if constexpr (PP::NF == 0) return;
@@ -395,6 +404,7 @@ __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            const Map &map) {
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
+  t = cp_opaque(t);
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
   bool prev_local = false, prev_pad = false;
 #pragma nounroll

@@ -40,6 +40,15 @@ __global__ void __launch_bounds__(256)
 
 __device__ __forceinline__ int se_byte(uint32_t w, int k) { return (int)(int8_t)(w >> (8 * k)); }
 
+// Wave-uniform pointer, kept in SGPRs.
+template <class P>
+__device__ __forceinline__ P *enc_uniform(P *q) {
+  const uint64_t v = (uint64_t)q;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (P *)(((uint64_t)hi << 32) | lo);
+}
+
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_sample_muladd(cp_gi *__restrict__ out, const cp_gi *__restrict__ a, int nunits, int nres,
@@ -61,14 +70,21 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const uint64_t p = pr.p;
   const uint64_t *twf = twf_all + (size_t)rid * N;
   const uint64_t *tsf = tsf_all + (size_t)rid * N * 6;
-  const size_t off = (size_t)unit * N;
   const uint32_t *se = se_all + inst * N;
+  const cp_gi *ap = a + (size_t)unit * N;
+  cp_gi *op = out + (size_t)unit * N;
+  if (ROWS == 1) {
+    se = enc_uniform(se);
+    ap = enc_uniform(ap);
+    op = enc_uniform(op);
+  }
   MapIdentity id;
   cp_gi x[EPT], y[EPT];
   // s: ternary, coefficient g in re and g + N in imag of slot g
+  const int ts = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
-    const int g = first_index<LOGN, 0, LOGN>(t, i);
+    const int g = first_index<LOGN, 0, LOGN>(ts, i);
     const uint32_t w = se[g];
     x[i].re = small_mod(se_byte(w, 0), p);
     x[i].imag = small_mod(se_byte(w, 1), p);
@@ -76,13 +92,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
-  // a * s at the spectrum positions this thread holds (last-pass layout)
+  // a * s at the spectrum positions this thread holds (last-pass layout),
+  // one slot's load in flight at a time
+  const int ta = cp_opaque(t);
 #pragma unroll
-  for (int i = 0; i < EPT; i++)
-    x[i] = cp_gimul(a[off + pass_index<LOGN, LP::LS0, LP::RL>(t, i)], x[i], pr);
+  for (int i = 0; i < EPT; i++) {
+    x[i] = cp_gimul(ap[pass_index<LOGN, LP::LS0, LP::RL>(ta, i)], x[i], pr);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   // e: discrete Gaussian
-  int te = t;
-  asm volatile("" : "+v"(te));
+  const int te = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<LOGN, 0, LOGN>(te, i);
@@ -93,13 +112,14 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(y, lds, t, twf, p, id);
+  const int to = cp_opaque(t);
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       cp_gi r;
       r.re = cp_add(x[i].re, y[i].re, p);
       r.imag = cp_add(x[i].imag, y[i].imag, p);
-      out[off + pass_index<LOGN, LP::LS0, LP::RL>(te, i)] = r;
+      op[pass_index<LOGN, LP::LS0, LP::RL>(to, i)] = r;
     }
   }
 }

@@ -253,17 +253,10 @@ __device__ __forceinline__ P *uniform_ptr(P *q) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
   return (P *)(((uint64_t)hi << 32) | lo);
 }
-// An opaque copy of the thread index: the address arithmetic that uses it
-// cannot be hoisted above this point (or shared between transforms and held
-// in VGPRs across them).
-__device__ __forceinline__ int opq(int t) {
-  asm volatile("" : "+v"(t));
-  return t;
-}
 template <int LOGN>
 __device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restrict__ src, int t,
                                          const uint64_t *__restrict__ tsf, uint64_t p) {
-  t = opq(t);
+  t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) x[i] = src[first_index<LOGN, 0, LOGN>(t, i)];
 #pragma unroll
@@ -276,7 +269,7 @@ template <int LOGN>
 __device__ __forceinline__ void fti_store(cp_gi *__restrict__ dst, cp_gi (&x)[EPT], int t,
                                           const uint64_t *__restrict__ tsi, uint64_t p,
                                           bool active) {
-  t = opq(t);
+  t = cp_opaque(t);
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -339,28 +332,28 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
   fti_load<LOGN>(xa, c0, t, tsf, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twf, p, id);  // S_c0, held
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c0, held
   fti_load<LOGN>(xb, d0, t, tsf, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twf, p, id);  // S_d0
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d0
   if (active) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) y1[i * T + opq(t)] = xb[i];
+    for (int i = 0; i < EPT; i++) y1[i * T + cp_opaque(t)] = xb[i];
   }
 #pragma unroll
   for (int i = 0; i < EPT; i++) xb[i] = cp_gimul(xa[i], xb[i], pr);
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twi, p, id);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twi, p, id);
   fti_store<LOGN>(y0, xb, t, tsi, p, active);
   fti_load<LOGN>(xb, d1, t, tsf, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twf, p, id);  // S_d1, held
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d1, held
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const cp_gi w = cp_gimul(xa[i], xb[i], pr);
-    if (active) y2[i * T + opq(t)] = w;
+    if (active) y2[i * T + cp_opaque(t)] = w;
     __builtin_amdgcn_sched_barrier(0);
   }
   fti_load<LOGN>(xa, c1, t, tsf, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twf, p, id);  // S_c1
-  const int tp = opq(t);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c1
+  const int tp = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     // one slot's park loads in flight at a time (all 16 hoisted would spill)
@@ -372,9 +365,9 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     xa[i].imag = cp_add(w.imag, pc0d1.imag, p);
     __builtin_amdgcn_sched_barrier(0);
   }
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, opq(t), twi, p, id);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twi, p, id);
   fti_store<LOGN>(y1, xa, t, tsi, p, active);
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, opq(t), twi, p, id);
+  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twi, p, id);
   fti_store<LOGN>(y2, xb, t, tsi, p, active);
 }
 

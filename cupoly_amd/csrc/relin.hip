@@ -131,11 +131,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- fused form, n = 2^11 .. 2^14 ---------------------------------------------
-__device__ __forceinline__ int relin_opq(int t) {
-  asm volatile("" : "+v"(t));
-  return t;
-}
-
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_relin_fti(cp_rcts out, const cp_gi *__restrict__ T, const cp_gi *__restrict__ key,
@@ -166,7 +161,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   cp_gi x[EPT], acc[EPT];
 #pragma nounroll
   for (int i = 0; i < k; i++) {
-    const int tl = relin_opq(t);
+    const int tl = cp_opaque(t);
     const cp_gi *y2i = y + ((size_t)2 * k + i) * N;
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
@@ -176,9 +171,9 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
       twist(x[r], tsf, g, p);
       __builtin_amdgcn_sched_barrier(0);
     }
-    dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, relin_opq(t), twf, p, id);
+    dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
     const cp_gi *kb = key + ((size_t)(2 * i) * k + j) * N, *ka = key + ((size_t)(2 * i + 1) * k + j) * N;
-    const int tk = relin_opq(t);
+    const int tk = cp_opaque(t);
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
       const int pos = pass_index<LOGN, LP::LS0, LP::RL>(tk, r);
@@ -199,12 +194,12 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 #pragma unroll
   for (int m = 0; m < 2; m++) {
     if (m == 1) {
-      const int tl = relin_opq(t);
+      const int tl = cp_opaque(t);
 #pragma unroll
       for (int r = 0; r < EPT; r++) acc[r] = park[r * TT + tl];
     }
-    dit_passes<LOGN, LOGN, 0, LOGN, 0>(acc, lds, relin_opq(t), twi, p, id);
-    const int ts = relin_opq(t);
+    dit_passes<LOGN, LOGN, 0, LOGN, 0>(acc, lds, t, twi, p, id);
+    const int ts = cp_opaque(t);
     if (active) {
       const cp_gi *ym = y + ((size_t)m * k + j) * N;
       cp_gi *om = o + ((size_t)m * k + j) * N;
