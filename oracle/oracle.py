@@ -459,3 +459,42 @@ def bfv_decrypt(P, ct, s_vals):
     s = fold(small_to_residues(s_vals, P.q), P.Ng)
     x = pointwise(P, u64(ct[0]), polymul_batch(P, u64(ct[1])[None], s[None])[0], op=0)
     return unfold(simple_scaling(P, x)).astype(np.int64)
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the optimised CPU port (cpu_port.c), timed by the benchmark.
+# --------------------------------------------------------------------------
+_port = None
+_port_plans = {}
+
+
+def port_lib():
+    global _port
+    if _port is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcpuport.so")
+        L = ctypes.CDLL(path)
+        L.cpu_plan_create.restype = ctypes.c_void_p
+        L.cpu_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _u64p, _u64p, _u64p, _u64p]
+        L.cpu_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.cpu_polymul_batch.restype = ctypes.c_int
+        L.cpu_polymul_batch.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, ctypes.c_int]
+        _port = L
+    return _port
+
+
+def port_polymul_batch(P, a, b):
+    """Coefficient-domain products c[i] = a[i] * b[i] (GI [batch, k, Ng, 2]) by
+    the optimised CPU port."""
+    key = (P.k, P.Ng)
+    if key not in _port_plans:
+        xi, xiinv, om = P.base_roots(0)
+        h = port_lib().cpu_plan_create(P.Ng, P.k, _p(u64(P.q)), _p(u64(xi).reshape(-1)),
+                                       _p(u64(xiinv).reshape(-1)), _p(u64(om)))
+        if not h:
+            raise MemoryError("cpu_port: plan allocation failed")
+        _port_plans[key] = h
+    a, b = np.ascontiguousarray(u64(a)), np.ascontiguousarray(u64(b))
+    c = np.empty_like(a)
+    if port_lib().cpu_polymul_batch(_port_plans[key], _p(c), _p(a), _p(b), a.shape[0]) != 0:
+        raise MemoryError("cpu_port: working buffer allocation failed")
+    return c

@@ -673,3 +673,17 @@ def test_oracle_bfv_multiply_relinearize_decrypts():
             acc = O.pointwise(P, O.dgt(P, Di, 0, 0), key[i, m], acc, op=3)
         wrong.append(O.pointwise(P, y[m], O.dgt(P, acc, 0, 1), op=0))
     assert not np.array_equal(O.bfv_decrypt(P, np.stack(wrong), s_vals), want)
+
+
+@pytest.mark.parametrize("n,k,batch", [(64, 2, 3), (2048, 3, 2), (16384, 4, 1)])
+def test_cpu_port_matches_oracle(n, k, batch):
+    """The CPU baseline (oracle/cpu_port.c, precomputed Shoup tables, no
+    division in its loops) computes the same products as the oracle."""
+    P = O.Params(k, k + 1, n, 65537)
+    rng = np.random.default_rng(n + k)
+    a = np.empty((batch, k, P.Ng, 2), dtype=np.uint64)
+    b = np.empty_like(a)
+    for j, qj in enumerate(P.q):
+        a[:, j] = rng.integers(0, qj, size=(batch, P.Ng, 2), dtype=np.uint64)
+        b[:, j] = rng.integers(0, qj, size=(batch, P.Ng, 2), dtype=np.uint64)
+    assert np.array_equal(O.port_polymul_batch(P, a, b), O.polymul_batch(P, a, b))
