@@ -5,8 +5,8 @@
 // coefficient j + n/2 -> imag of slot j, each reduced mod every prime), and
 // poly_icrt (polynomial.cu:221-291) rebuilds
 //     x = sum_i (Q/q_i) [x_i (Q/q_i)^-1]_{q_i}  mod Q
-// with host big integers.  Here both run one thread per coefficient on
-// little-endian 64-bit limbs:
+// with host big integers.  Here both run on little-endian 64-bit limbs,
+// crt with one thread per (slot, residue), icrt with one per coefficient:
 //   crt:  x mod p by Horner from the top limb, r <- r (2^64 mod p) + w, with a
 //         Shoup product by the per-prime constant 2^64 mod p;
 //   icrt: the sum in k+1 limbs (128-bit multiply-accumulate against the limbs
@@ -28,23 +28,22 @@ __device__ __forceinline__ uint64_t limbs_mod(const uint64_t *__restrict__ x, in
   return r;
 }
 
-// coefs: [ncoefs][limbs]; out: [nres][ng] GIs of residues roff..roff+nres-1
+// coefs: [ncoefs][limbs]; out: [nres][ng] GIs of residues roff..roff+nres-1;
+// one thread per (slot, residue r = blockIdx.y)
 __global__ void __launch_bounds__(256)
     k_crt(cp_gi *__restrict__ out, const uint64_t *__restrict__ coefs, int ncoefs, int limbs,
-          int ng, int nres, int roff, const cp_crt *__restrict__ C,
-          const cp_prime *__restrict__ primes) {
+          int ng, int roff, const cp_crt *__restrict__ C, const cp_prime *__restrict__ primes) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= ng) return;
+  const int r = blockIdx.y;
   const bool has0 = slot < ncoefs, has1 = slot + ng < ncoefs;
   const uint64_t *x0 = coefs + (size_t)slot * limbs, *x1 = coefs + (size_t)(slot + ng) * limbs;
-  for (int r = 0; r < nres; r++) {
-    const uint64_t q = primes[roff + r].p;
-    const uint64_t c = C->two64[roff + r][0], cp = C->two64[roff + r][1];
-    cp_gi v;
-    v.re = has0 ? limbs_mod(x0, limbs, q, c, cp) : 0;
-    v.imag = has1 ? limbs_mod(x1, limbs, q, c, cp) : 0;
-    out[(size_t)r * ng + slot] = v;
-  }
+  const uint64_t q = primes[roff + r].p;
+  const uint64_t c = C->two64[roff + r][0], cp = C->two64[roff + r][1];
+  cp_gi v;
+  v.re = has0 ? limbs_mod(x0, limbs, q, c, cp) : 0;
+  v.imag = has1 ? limbs_mod(x1, limbs, q, c, cp) : 0;
+  out[(size_t)r * ng + slot] = v;
 }
 
 // in: [K][ng] GIs of base Q; out: [2 ng][K] limbs of x in [0, Q).
@@ -128,8 +127,8 @@ extern "C" int cupoly_crt(cupoly_ctx *c, cupoly_gi *out, const uint64_t *coefs, 
   if (ncoefs > 2 * ng) return cp_fail(CUPOLY_ERR_ARG, "crt: more coefficients than the ring degree");
   int off, cnt;
   if (!cp_base_range(c->e, base, &off, &cnt)) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "Unknown base");
-  hipLaunchKernelGGL(k_crt, dim3((ng + 255) / 256), dim3(256), 0, c->stream, (cp_gi *)out, coefs,
-                     ncoefs, limbs, ng, cnt, off, c->e->d_crt, c->e->d_primes);
+  hipLaunchKernelGGL(k_crt, dim3((ng + 255) / 256, cnt), dim3(256), 0, c->stream, (cp_gi *)out,
+                     coefs, ncoefs, limbs, ng, off, c->e->d_crt, c->e->d_primes);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }

@@ -78,34 +78,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Complex scaling t/Q, residue i of one slot (cudaengine.cu:401-433): the
-// reference forms
-//     y_i = sum_j (x_j w_ij mod b_i  (+)  h_j)  (+)  round  (+)  lambda_i xB_i mod b_i
-// with its single-correction addmod (+), where h_j = floor(x_j omega_frac_j
-// / 2^128) < q_j and round = llrint(sum_j RN(l_j 2^-64)) <= k.  When every
-// h_j < b_i, every operand of the chain is below b_i, the chain is plain
-// modular addition, and the exact sum reduced once (a 160-bit accumulation,
-// hsum = sum_j h_j + round added as its starting value) is the same word.
-// An h_j >= b_i needs x_j within 2^30 of q_j (the primes lie within 2^30 of
-// 2^63): about one coefficient in 2^33.  Such a slot takes the reference's
-// chain as written (`slow`), so the result is the same word in every case.
-__device__ __forceinline__ uint64_t scale_exact(const cp_acc &hsum, const uint64_t *x, int kq,
-                                                const uint64_t (*wint)[2], uint64_t xb,
-                                                const uint64_t (&lam)[2], uint64_t b) {
-  cp_acc a = hsum;
-#pragma unroll
-  for (int j = 0; j < kq; j++) cp_acc_mac(a, x[j], wint[j][0]);
-  cp_acc_mac(a, xb, lam[0]);
-  return cp_acc_reduce(a, b);
-}
-
-// hsum = sum_j h_j + round as a 160-bit accumulator value
-__device__ __forceinline__ void hsum_add(cp_acc &s, uint64_t v) {
-  unsigned c = 0;
-  s.lo = cp_addc64(s.lo, v, c);
-  s.hi += c;
-}
-
 // Complex scaling t/Q (QB -> B) and B -> Q in registers, K = k known at
 // compile time.  blockIdx.y = output polynomial index (pair * 3 + m).  The
 // fraction products x_j omega_frac_j and their FP64 sum do not depend on the
@@ -173,17 +145,8 @@ __global__ void __launch_bounds__(256)
       rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
       ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
     } else {  // the reference's chain, operation for operation
-      rr = 0;
-      ri = 0;
-#pragma unroll
-      for (int j = 0; j < K; j++) {
-        rr = addmod_ref(rr, addmod_ref(shoup2(xr[j], H->wint[i][j], b), fr[j], b), b);
-        ri = addmod_ref(ri, addmod_ref(shoup2(xi[j], H->wint[i][j], b), fi[j], b), b);
-      }
-      rr = addmod_ref(rr, round_r, b);
-      ri = addmod_ref(ri, round_i, b);
-      rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
-      ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+      rr = scale_chain_word(xr, fr, K, H->wint[i], round_r, yv.re, H->lam[i], b);
+      ri = scale_chain_word(xi, fi, K, H->wint[i], round_i, yv.imag, H->lam[i], b);
     }
     // B -> Q, term i (cudaengine.cu:688-703)
     const uint64_t ar = shoup2(rr, H->invBbi[i], b), ai = shoup2(ri, H->invBbi[i], b);
@@ -471,17 +434,8 @@ __global__ void __launch_bounds__(256)
     rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
     ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
   } else {
-    rr = 0;
-    ri = 0;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      rr = addmod_ref(rr, addmod_ref(shoup2(xr[j], H->wint[i][j], b), fr[j], b), b);
-      ri = addmod_ref(ri, addmod_ref(shoup2(xi[j], H->wint[i][j], b), fi[j], b), b);
-    }
-    rr = addmod_ref(rr, round_r, b);
-    ri = addmod_ref(ri, round_i, b);
-    rr = addmod_ref(rr, shoup2(yv.re, H->lam[i], b), b);
-    ri = addmod_ref(ri, shoup2(yv.imag, H->lam[i], b), b);
+    rr = scale_chain_word(xr, fr, K, H->wint[i], round_r, yv.re, H->lam[i], b);
+    ri = scale_chain_word(xi, fi, K, H->wint[i], round_i, yv.imag, H->lam[i], b);
   }
   Z[((size_t)pm * kl + i) * ng + c] = cp_gi{shoup2(rr, H->invBbi[i], b), shoup2(ri, H->invBbi[i], b)};
 }
@@ -497,7 +451,7 @@ __global__ void __launch_bounds__(256)
   const uint64_t q = H->q[j].p;
   const cp_gi *z = Z + (size_t)pm * kl * ng;
   double vr = 0.0, vi = 0.0;
-  cp_acc sr, si;  // exact sums, reduced once (as hps_b_to_q_slot)
+  cp_acc sr, si;  // exact sums (every term reduced), reduced once
   cp_acc_init(sr);
   cp_acc_init(si);
   for (int i = 0; i < kl; i++) {  // ascending i, as the per-slot form

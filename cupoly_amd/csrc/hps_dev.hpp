@@ -94,97 +94,44 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
   }
 }
 
-// Complex scaling t/Q for slot c: aq[j*qs + c] (j < k), ab[i*bs + c] (i < kl)
-// -> yr[i], yi[i] (residues mod b_i of the re / imag coefficient).
-// omega_frac_j does not depend on the output residue i (it is t (QB/q_j)^-1
-// mod q_j B / q_j, reduced to its fraction), so the 192-bit products x_j
-// omega_frac_j and the FP64 sum of their low words are formed once per slot
-// and reused for every i; the sum runs over j in the same order as the
-// reference's per-(i, slot) thread, so it is the same double.
-__device__ __forceinline__ void hps_scale_slot(const cp_gi *__restrict__ aq, size_t qs,
-                                               const cp_gi *__restrict__ ab, size_t bs, int c,
-                                               const cp_hps *__restrict__ H,
-                                               uint64_t (&yr)[CP_MAX_BASE],
-                                               uint64_t (&yi)[CP_MAX_BASE]) {
-  const int k = H->k, kl = H->kl;
-  cp_gi x[CP_MAX_BASE];
-  uint64_t hr[CP_MAX_BASE], hi[CP_MAX_BASE];
-  const double two_m64 = 0x1p-64;  // __drcp_rn(2^64)
-  double acc_r = 0.0, acc_i = 0.0;
+// Complex scaling t/Q, residue i of one slot (cudaengine.cu:401-433): the
+// reference forms
+//     y_i = sum_j (x_j w_ij mod b_i  (+)  h_j)  (+)  round  (+)  lambda_i xB_i mod b_i
+// with its single-correction addmod (+), where h_j = floor(x_j omega_frac_j
+// / 2^128) < q_j and round = llrint(sum_j RN(l_j 2^-64)) <= k.  When every
+// h_j < b_i, every operand of the chain is below b_i, the chain is plain
+// modular addition, and the exact sum reduced once (a 160-bit accumulation,
+// hsum = sum_j h_j + round added as its starting value) is the same word.
+// An h_j >= b_i needs x_j within 2^30 of q_j (the primes lie within 2^30 of
+// 2^63): about one coefficient in 2^33.  Such a slot takes the reference's
+// chain as written (`slow`), so the result is the same word in every case.
+__device__ __forceinline__ uint64_t scale_exact(const cp_acc &hsum, const uint64_t *x, int kq,
+                                                const uint64_t (*wint)[2], uint64_t xb,
+                                                const uint64_t (&lam)[2], uint64_t b) {
+  cp_acc a = hsum;
 #pragma unroll
-  for (int j = 0; j < CP_MAX_BASE; j++) {
-    if (j < k) {
-      x[j] = aq[(size_t)j * qs + c];
-      uint64_t lr, li;
-      umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].re, lr, hr[j]);
-      umul128hi(H->wfrac[j][0], H->wfrac[j][1], x[j].imag, li, hi[j]);
-      acc_r = fma((double)lr, two_m64, acc_r);  // __fma_rn (cudaengine.cu:413-414)
-      acc_i = fma((double)li, two_m64, acc_i);
-    }
-  }
-  const uint64_t round_r = (uint64_t)llrint(acc_r), round_i = (uint64_t)llrint(acc_i);
-#pragma unroll
-  for (int i = 0; i < CP_MAX_BASE; i++) {
-    if (i < kl) {
-      const uint64_t b = H->b[i].p;
-      uint64_t rr = 0, ri = 0;
-#pragma unroll
-      for (int j = 0; j < CP_MAX_BASE; j++) {
-        if (j < k) {
-          const uint64_t int_r = shoup2(x[j].re, H->wint[i][j], b);
-          const uint64_t int_i = shoup2(x[j].imag, H->wint[i][j], b);
-          rr = addmod_ref(rr, addmod_ref(int_r, hr[j], b), b);
-          ri = addmod_ref(ri, addmod_ref(int_i, hi[j], b), b);
-        }
-      }
-      rr = addmod_ref(rr, round_r, b);
-      ri = addmod_ref(ri, round_i, b);
-      const cp_gi y = ab[(size_t)i * bs + c];
-      yr[i] = addmod_ref(rr, shoup2(y.re, H->lam[i], b), b);
-      yi[i] = addmod_ref(ri, shoup2(y.imag, H->lam[i], b), b);
-    }
-  }
+  for (int j = 0; j < kq; j++) cp_acc_mac(a, x[j], wint[j][0]);
+  cp_acc_mac(a, xb, lam[0]);
+  return cp_acc_reduce(a, b);
 }
 
-// B -> Q for slot c from residues xr[i], xi[i] mod b_i -> out[j*qs + c] (j < k).
-__device__ __forceinline__ void hps_b_to_q_slot(const uint64_t (&xr)[CP_MAX_BASE],
-                                                const uint64_t (&xi)[CP_MAX_BASE], cp_gi *out,
-                                                size_t qs, int c, const cp_hps *__restrict__ H) {
-  const int k = H->k, kl = H->kl;
-  uint64_t ar[CP_MAX_BASE], ai[CP_MAX_BASE];
-  double vr = 0.0, vi = 0.0;
-  // v = RN(v + RN(aux_i) / RN(b_i)) in ascending i (cudaengine.cu:699-700)
+// hsum = sum_j h_j + round as a 160-bit accumulator value
+__device__ __forceinline__ void hsum_add(cp_acc &s, uint64_t v) {
+  unsigned c = 0;
+  s.lo = cp_addc64(s.lo, v, c);
+  s.hi += c;
+}
+
+// The reference's chain for residue i of one coefficient, operation for
+// operation (cudaengine.cu:401-433): x[j] the Q words, h[j] their fraction
+// products' high words, round = llrint of the FP64 sum of the low words.
+__device__ __forceinline__ uint64_t scale_chain_word(const uint64_t *x, const uint64_t *h, int kq,
+                                                     const uint64_t (*wint)[2], uint64_t round,
+                                                     uint64_t xb, const uint64_t (&lam)[2],
+                                                     uint64_t b) {
+  uint64_t r = 0;
 #pragma unroll
-  for (int i = 0; i < CP_MAX_BASE; i++) {
-    if (i < kl) {
-      const uint64_t b = H->b[i].p;
-      ar[i] = shoup2(xr[i], H->invBbi[i], b);
-      ai[i] = shoup2(xi[i], H->invBbi[i], b);
-      const double bd = (double)b;
-      vr = vr + (double)ar[i] / bd;
-      vi = vi + (double)ai[i] / bd;
-    }
-  }
-  const uint64_t v_re = snap(vr), v_im = snap(vi);
-  for (int j = 0; j < k; j++) {
-    // (sum_i a_i (B/b_i mod q_j) - v (B mod q_j)) mod q_j, reduced once
-    const uint64_t q = H->q[j].p, bq = H->Bqi[j][0];
-    const uint64_t nbq = bq ? q - bq : 0;  // -B mod q_j
-    cp_acc sr, si;
-    cp_acc_init(sr);
-    cp_acc_init(si);
-#pragma unroll
-    for (int i = 0; i < CP_MAX_BASE; i++) {
-      if (i < kl) {
-        cp_acc_mac(sr, ar[i], H->Bbiqi[j][i][0]);
-        cp_acc_mac(si, ai[i], H->Bbiqi[j][i][0]);
-      }
-    }
-    cp_acc_mac(sr, v_re, nbq);
-    cp_acc_mac(si, v_im, nbq);
-    cp_gi o;
-    o.re = cp_acc_reduce(sr, q);
-    o.imag = cp_acc_reduce(si, q);
-    out[(size_t)j * qs + c] = o;
-  }
+  for (int j = 0; j < kq; j++) r = addmod_ref(r, addmod_ref(shoup2(x[j], wint[j], b), h[j], b), b);
+  r = addmod_ref(r, round, b);
+  return addmod_ref(r, shoup2(xb, lam, b), b);
 }

@@ -77,97 +77,126 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// One workgroup: the h coefficients (of n = 2 ng) with the smallest keys
-// become +-1, all others 0; every residue gets the value mod q_r.  The h-th
-// smallest key is found by radix selection, 8 bits per pass from the top.
-// Selection stops as soon as every key sharing the resolved top bits with
-// the h-th one is itself selected (its bucket holds exactly the remaining
-// rank, e.g. a bucket of one key): then "top bits <= the resolved prefix"
-// picks the same h keys as a full 64-bit comparison, usually after 2 or 3
-// passes instead of 8.  With KPT > 0 each thread keeps its KPT keys (and
-// signs) in registers, so Philox runs once per coefficient; KPT = 0 (n above
-// 16384) recomputes them in each pass.
+// Fixed weight (ZO, Hamming weight): the h coefficients (of n = 2 ng) with
+// the smallest keys become +-1, all others 0; every residue gets the value
+// mod q_r.  Three launches:
+//   1. k_fw_keys:   every coefficient's key (one Philox draw), in parallel,
+//                   into the context's scratch;
+//   2. k_fw_select: one workgroup finds the h-th smallest key by radix
+//                   selection, 8 bits per pass from the top, each pass a
+//                   256-bucket histogram and a parallel scan of it.
+//                   Selection stops as soon as every key sharing the
+//                   resolved top bits with the h-th one is itself selected
+//                   (its bucket holds exactly the remaining rank, e.g. a
+//                   bucket of one key): then "top bits <= the resolved
+//                   prefix" picks the same h keys as a full 64-bit
+//                   comparison, usually after 2 or 3 passes instead of 8.
+//                   Writes (bound, shift);
+//   3. k_fw_write:  every coefficient redraws its key and sign and writes its
+//                   value to every residue, in parallel.
+// The keys are distinct (the coefficient index is in their low bits), so the
+// selected set is the same for any launch shape.
 #define FW_THREADS 1024
+
+__global__ void __launch_bounds__(256)
+    k_fw_keys(uint64_t *__restrict__ keys, int n, uint64_t seed, uint64_t stream) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  bool neg;
+  keys[j] = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+}
+
+// sel[0] = bound (resolved top bits of the h-th key), sel[1] = shift; h in
+// (0, n).  With KPT > 0 (n <= KPT * FW_THREADS) every thread loads its KPT
+// keys once, all loads in flight together, and keeps them in registers for
+// every pass; KPT = 0 reads them again in each pass, 16 loads at a time.
 template <int KPT>
 __global__ void __launch_bounds__(FW_THREADS)
-    k_sample_fixed_weight(cp_gi *__restrict__ out, int ng, int nres, int roff,
-                          const cp_prime *__restrict__ primes, uint64_t seed, uint64_t stream,
-                          int h) {
+    k_fw_select(uint64_t *__restrict__ sel, const uint64_t *__restrict__ keys, int n, int h) {
   __shared__ unsigned hist[256];
+  __shared__ unsigned scan[256];
   __shared__ uint64_t s_prefix;
   __shared__ int s_need, s_done;
-  const int n = 2 * ng, tid = (int)threadIdx.x;
-  uint64_t keys[KPT > 0 ? KPT : 1];
-  uint32_t negs = 0;
+  const int tid = (int)threadIdx.x;
+  constexpr int CH = KPT > 0 ? KPT : 16;
+  uint64_t kr[CH];
   if constexpr (KPT > 0) {
 #pragma unroll
     for (int u = 0; u < KPT; u++) {
       const int j = tid + u * FW_THREADS;
-      bool neg = false;
-      keys[u] = j < n ? fixed_weight_key(seed, stream, (uint32_t)j, neg) : ~0ull;
-      negs |= (uint32_t)neg << u;
+      kr[u] = j < n ? keys[j] : 0;
     }
   }
   uint64_t prefix = 0;  // top bits of the h-th smallest key, 8 more per pass
   int need = h;         // its rank among the keys that share those bits
   int shift = 0;        // low bits not resolved when selection stopped
-  if (h > 0 && h < n) {
-    for (int pass = 0; pass < 8; pass++) {
-      shift = 56 - 8 * pass;
-      for (int i = tid; i < 256; i += FW_THREADS) hist[i] = 0;
-      __syncthreads();
-      if constexpr (KPT > 0) {
+  for (int pass = 0; pass < 8; pass++) {
+    shift = 56 - 8 * pass;
+    if (tid < 256) hist[tid] = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += CH * FW_THREADS) {
+      if constexpr (KPT == 0) {
 #pragma unroll
-        for (int u = 0; u < KPT; u++) {
-          const uint64_t key = keys[u];
-          if (tid + u * FW_THREADS < n &&
-              (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8))))
-            atomicAdd(&hist[(key >> shift) & 255], 1u);
-        }
-      } else {
-        for (int j = tid; j < n; j += FW_THREADS) {
-          bool neg;
-          const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
-          if (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
-            atomicAdd(&hist[(key >> shift) & 255], 1u);
+        for (int u = 0; u < CH; u++) {
+          const int j = base + tid + u * FW_THREADS;
+          kr[u] = j < n ? keys[j] : 0;
         }
       }
-      __syncthreads();
-      if (tid == 0) {
-        int cum = 0, d = 0;
-        for (; d < 255; d++) {
-          if (cum + (int)hist[d] >= need) break;
-          cum += (int)hist[d];
-        }
-        s_need = need - cum;
-        s_done = (need - cum) == (int)hist[d];
-        s_prefix = prefix | ((uint64_t)d << shift);
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int j = base + tid + u * FW_THREADS;
+        const uint64_t key = kr[u];
+        if (j < n && (pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8))))
+          atomicAdd(&hist[(key >> shift) & 255], 1u);
       }
-      __syncthreads();
-      prefix = s_prefix;
-      need = s_need;
-      const bool done = s_done != 0;
-      __syncthreads();  // hist and the broadcast slots are reused by the next pass
-      if (done) break;  // uniform
+      if constexpr (KPT > 0) break;  // every key is in registers
     }
+    __syncthreads();
+    // inclusive scan of the 256 buckets (Hillis-Steele, one thread per bucket)
+    unsigned v = tid < 256 ? hist[tid] : 0;
+    for (int d = 1; d < 256; d <<= 1) {
+      if (tid < 256) scan[tid] = v;
+      __syncthreads();
+      if (tid < 256 && tid >= d) v += scan[tid - d];
+      __syncthreads();
+    }
+    // the bucket holding rank `need`: cumulative count before it < need <= through it
+    if (tid < 256) {
+      const int incl = (int)v, excl = incl - (int)hist[tid];
+      if (excl < need && need <= incl) {
+        s_need = need - excl;
+        s_done = (need - excl) == (int)hist[tid];
+        s_prefix = prefix | ((uint64_t)tid << shift);
+      }
+    }
+    __syncthreads();
+    prefix = s_prefix;
+    need = s_need;
+    const bool done = s_done != 0;
+    __syncthreads();  // hist, scan and the broadcast slots are reused by the next pass
+    if (done) break;  // uniform
   }
-  const uint64_t bound = prefix >> shift;
-  for (int u = 0, j = tid; j < n; u++, j += FW_THREADS) {
-    bool neg;
-    uint64_t key;
-    if constexpr (KPT > 0) {
-      key = keys[u < KPT ? u : 0];
-      neg = (negs >> u) & 1;
-    } else {
-      key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
-    }
-    const bool sel = h >= n || (h > 0 && (key >> shift) <= bound);
-    const int e = sel ? (neg ? -1 : 1) : 0;
-    const int slot = j < ng ? j : j - ng;
-    for (int r = 0; r < nres; r++) {
-      uint64_t *dst = j < ng ? &out[(size_t)r * ng + slot].re : &out[(size_t)r * ng + slot].imag;
-      *dst = small_mod(e, primes[roff + r].p);
-    }
+  if (tid == 0) {
+    sel[0] = prefix >> shift;
+    sel[1] = (uint64_t)shift;
+  }
+}
+
+// sel == nullptr: h = 0 (no coefficient selected) or h >= n (all selected)
+__global__ void __launch_bounds__(256)
+    k_fw_write(cp_gi *__restrict__ out, int ng, int nres, int roff,
+               const cp_prime *__restrict__ primes, uint64_t seed, uint64_t stream,
+               const uint64_t *__restrict__ sel, bool all) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= 2 * ng) return;
+  bool neg;
+  const uint64_t key = fixed_weight_key(seed, stream, (uint32_t)j, neg);
+  const bool on = sel ? (key >> sel[1]) <= sel[0] : all;
+  const int e = on ? (neg ? -1 : 1) : 0;
+  const int slot = j < ng ? j : j - ng;
+  for (int r = 0; r < nres; r++) {
+    uint64_t *dst = j < ng ? &out[(size_t)r * ng + slot].re : &out[(size_t)r * ng + slot].imag;
+    *dst = small_mod(e, primes[roff + r].p);
   }
 }
 
@@ -207,15 +236,28 @@ extern "C" int cupoly_sample_coeff(cupoly_ctx *c, cupoly_gi *out, int kind, uint
       break;
     case CUPOLY_ZO:
     case CUPOLY_HAMMINGWEIGHT: {
-      if (2 * ng > 65536) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fixed-weight sampler: n > 65536");
-      const int h = kind == CUPOLY_ZO ? ng : (2 * ng < CUPOLY_HAMMINGWEIGHT_H ? 2 * ng
-                                                                           : CUPOLY_HAMMINGWEIGHT_H);
-      if (2 * ng <= 16 * FW_THREADS)
-        hipLaunchKernelGGL(k_sample_fixed_weight<16>, dim3(1), dim3(FW_THREADS), 0, c->stream,
-                           (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
-      else
-        hipLaunchKernelGGL(k_sample_fixed_weight<0>, dim3(1), dim3(FW_THREADS), 0, c->stream,
-                           (cp_gi *)out, ng, cnt, off, c->e->d_primes, seed, stream, h);
+      const int n = 2 * ng;
+      if (n > 65536) return cp_fail(CUPOLY_ERR_UNSUPPORTED, "fixed-weight sampler: n > 65536");
+      const int h = kind == CUPOLY_ZO ? ng : (n < CUPOLY_HAMMINGWEIGHT_H ? n
+                                                                        : CUPOLY_HAMMINGWEIGHT_H);
+      const dim3 g((n + 255) / 256);
+      if (h > 0 && h < n) {
+        // keys (n words) and the selection (2 words) in the context's scratch
+        // row d_tmp, which holds nres * ng >= n Gaussian slots
+        uint64_t *keys = (uint64_t *)c->d_tmp, *sel = keys + n;
+        hipLaunchKernelGGL(k_fw_keys, g, dim3(256), 0, c->stream, keys, n, seed, stream);
+        if (n <= 16 * FW_THREADS)
+          hipLaunchKernelGGL(k_fw_select<16>, dim3(1), dim3(FW_THREADS), 0, c->stream, sel,
+                             (const uint64_t *)keys, n, h);
+        else
+          hipLaunchKernelGGL(k_fw_select<0>, dim3(1), dim3(FW_THREADS), 0, c->stream, sel,
+                             (const uint64_t *)keys, n, h);
+        hipLaunchKernelGGL(k_fw_write, g, dim3(256), 0, c->stream, (cp_gi *)out, ng, cnt, off,
+                           c->e->d_primes, seed, stream, (const uint64_t *)sel, false);
+      } else {
+        hipLaunchKernelGGL(k_fw_write, g, dim3(256), 0, c->stream, (cp_gi *)out, ng, cnt, off,
+                           c->e->d_primes, seed, stream, (const uint64_t *)nullptr, h >= n);
+      }
       break;
     }
     default: return cp_fail(CUPOLY_ERR_ARG, "Invalid sampler: " + std::to_string(kind));

@@ -241,10 +241,11 @@ int cupoly_hom_mul_batch_noalloc(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi 
 /* Launch choices of the hom-mul pipeline on this context (no reference
  * counterpart): below `wide_slots` output slots (batch * 3 * n/2; 0 = the
  * built-in 65536) the HPS steps run one thread per (slot, residue);
- * scale_chain = 1 computes every t/Q residue with the reference's addmod
- * chain (cudaengine.cu:401-433) instead of the exact sums, which give the
- * same words (the default, 0, takes the chain only where the reference's
- * operands arrive unreduced). */
+ * scale_chain = 1 computes every t/Q residue (here and in
+ * cupoly_complex_scaling on this context) with the reference's addmod chain
+ * (cudaengine.cu:401-433) instead of the exact sums, which give the same
+ * words (the default, 0, takes the chain only where the reference's operands
+ * arrive unreduced). */
 int cupoly_hom_mul_set_tuning(cupoly_ctx *c, long wide_slots, int scale_chain);
 /* Device bytes of the hom-mul workspace for `batch` pairs of e's shape (no
  * reference counterpart; the reference sizes its Context scratch from the

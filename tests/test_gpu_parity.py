@@ -202,6 +202,18 @@ def test_hps_base_changes_at_extreme_residues(gpu, k):
     oq = cp.empty((k, P.Ng, 2))
     cp.execute_polynomial_basis_ext_B_to_Q(ctx, oq, cp.to_device(xb))
     assert np.array_equal(cp.to_host(oq), O.basis_ext_b_to_q(P, xb))
+    # complex scaling at the same extremes, with the exact sums and with the
+    # reference's addmod chain forced on every slot: both equal the oracle
+    xbq = O.fold(xb, P.Ng)
+    want = O.complex_scaling(P, xq, xbq)
+    out = cp.empty((kl, n))
+    try:
+        for chain in (0, 1):
+            cp.hom_mul_set_tuning(ctx, 0, chain)
+            cp.execute_polynomial_complex_scaling(ctx, out, cp.to_device(xq), cp.to_device(xbq))
+            assert np.array_equal(cp.to_host(out), want), chain
+    finally:
+        cp.hom_mul_set_tuning(ctx, 0, 0)
 
 
 @pytest.mark.parametrize("k,n,t", [(4, 16384, 65537), (3, 4096, 256), (2, 2048, 40961)])
