@@ -197,6 +197,106 @@ def bench_uneven(count, steps, seed=4096):
             "hbm_frac": nbytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+def bench_mixed(count, reps=10, seed=77):
+    """`count` relinearized BFV multiplications, 70% (n=2^13, k=3) and 30%
+    (n=2^15, k=8), uniform ciphertexts from the Philox sampler: one
+    mixed-shape plan (a single graph launch, one branch per shape) against
+    the two per-shape batched calls issued one after the other on one
+    stream.  Outputs are checked equal."""
+    import torch
+    import cupoly_amd as cp
+
+    dims = [(8192, 3), (32768, 8)]
+    c0 = int(round(0.7 * count))
+    counts = [c0, count - c0]
+    engines = [cp.CUDAEngine(k, k + 1, n, 65537) for n, k in dims]
+    stream = torch.cuda.Stream()
+    ctxs = [cp.Context(e, stream=stream) for e in engines]
+    keys, cts, outs, refs = [], [], [], []
+    for si, ((n, k), c) in enumerate(zip(dims, ctxs)):
+        ng = n // 2
+        sk = cp.empty((k, ng, 2))
+        with torch.cuda.stream(stream):
+            cp.Sampler.sample(c, sk, cp.TERNARY, 4242, si, coeff_domain=True)
+        keys.append(cp.RelinKey(c, sk, 4242, 1))
+        a = cp.empty((counts[si], 2, k, ng, 2))
+        b = cp.empty((counts[si], 2, k, ng, 2))
+        with torch.cuda.stream(stream):
+            for i in range(counts[si]):
+                for j in range(2):
+                    cp.Sampler.sample(c, a[i, j], cp.UNIFORM, seed + si, 4 * i + j, coeff_domain=True)
+                    cp.Sampler.sample(c, b[i, j], cp.UNIFORM, seed + si, 4 * i + 2 + j,
+                                      coeff_domain=True)
+        cts.append((a, b))
+        outs.append(cp.empty((counts[si], 2, k, ng, 2)))
+        refs.append(cp.empty((counts[si], 2, k, ng, 2)))
+    stream.synchronize()
+    reqs = [(si, outs[si][i], cts[si][0][i], cts[si][1][i])
+            for si in range(len(dims)) for i in range(counts[si])]
+    origin = cp.Context(engines[0], stream=stream)
+    plan = cp.HomMulPlan(origin, engines, reqs, keys=keys)
+
+    def per_shape():
+        for si in range(len(dims)):
+            cp.hom_mul_relin_batch(ctxs[si], keys[si], refs[si], cts[si][0], cts[si][1])
+
+    plan.run()
+    per_shape()  # warm-up: workspaces
+    stream.synchronize()
+    for si in range(len(dims)):
+        assert torch.equal(outs[si], refs[si]), "mixed plan and per-shape calls differ"
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    plan_ms = timed(plan.run)
+    seq_ms = timed(per_shape)
+    small = None
+    if count > 10:
+        # latency case: 7 + 3 requests, neither batch occupies every CU
+        sreqs = [r for r in reqs if r[0] == 0][:7] + [r for r in reqs if r[0] == 1][:3]
+        splan = cp.HomMulPlan(origin, engines, sreqs, keys=keys)
+        sa = [(torch.stack([r[2] for r in sreqs if r[0] == si]),
+               torch.stack([r[3] for r in sreqs if r[0] == si]),
+               cp.empty((7 if si == 0 else 3,) + tuple(outs[si].shape[1:])))
+              for si in range(len(dims))]
+
+        def small_per_shape():
+            for si in range(len(dims)):
+                cp.hom_mul_relin_batch(ctxs[si], keys[si], sa[si][2], sa[si][0], sa[si][1])
+
+        splan.run()
+        small_per_shape()
+        stream.synchronize()
+        for si in range(len(dims)):
+            got = torch.stack([r[1] for r in sreqs if r[0] == si])
+            assert torch.equal(got, sa[si][2]), "mixed plan (7 + 3) and per-shape calls differ"
+        small = {"count": 10, "ms_per_batch": timed(splan.run),
+                 "per_shape_calls_ms": timed(small_per_shape),
+                 "mix": "7 x (n=8192, k=3) + 3 x (n=32768, k=8), relinearized"}
+        small["speedup_vs_per_shape_calls"] = small["per_shape_calls_ms"] / small["ms_per_batch"]
+        splan.destroy()
+    nbytes = sum(48 * k * n * c for (n, k), c in zip(dims, counts))
+    plan.destroy()
+    for key in keys:
+        key.destroy()
+    for c in ctxs + [origin]:
+        c.destroy()
+    return {"value": count / (plan_ms * 1e-3), "unit": "hom-mults/s", "count": count,
+            "ms_per_batch": plan_ms, "per_shape_calls_ms": seq_ms,
+            "speedup_vs_per_shape_calls": seq_ms / plan_ms,
+            "mix": "%d x (n=8192, k=3) + %d x (n=32768, k=8), t=65537, relinearized" % tuple(counts),
+            "form": "one hipGraph launch, the shapes' pipelines as parallel branches",
+            "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (plan_ms * 1e-3) / 1e9,
+            "small": small}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +320,9 @@ def main():
                     help="offered load of the serving run, requests/s (Poisson)")
     ap.add_argument("--uneven-count", type=int, default=4096,
                     help="products in the uneven batch (n in 2^11..2^15, k in 1..8); 0 skips")
+    ap.add_argument("--mixed-count", type=int, default=100,
+                    help="relinearized multiplications in the mixed-shape plan (70%% n=2^13 "
+                         "k=3, 30%% n=2^15 k=8); 0 skips")
     ap.add_argument("--edges", type=int, default=1,
                     help="1: also time the coefficient-edge / key-switching / sampler ops")
     ap.add_argument("--no-cpu", action="store_true")
@@ -394,6 +497,12 @@ def main():
         print("bench: uneven done %.0f products/s (%.1fx plain)"
               % (uneven["value"], uneven["speedup_vs_plain"]), file=sys.stderr, flush=True)
 
+    mixed = None
+    if args.mixed_count > 0:
+        mixed = bench_mixed(args.mixed_count)
+        print("bench: mixed plan done %.0f hom-mults/s (%.2fx per-shape calls)"
+              % (mixed["value"], mixed["speedup_vs_per_shape_calls"]), file=sys.stderr, flush=True)
+
     edges = bench_edges(n, k) if args.edges else None
     if edges:
         print("bench: edges done (icrt %.1f us, crt %.1f us)" % (edges["icrt"]["us"], edges["crt"]["us"]),
@@ -502,6 +611,7 @@ def main():
         "sample_muladd": enc,
         "serving": serve,
         "uneven": uneven,
+        "mixed_hom_mul": mixed,
         "edges": edges,
     }
     print(json.dumps(line), flush=True)

@@ -81,6 +81,10 @@ SIGNATURES = {
     "cupoly_hom_mul_relin_batch": ([_vp, _vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_relin_gather": ([_vp, _vp, _vp, _vp, _vp, _int], _int),
     "cupoly_hom_mul_relin_workspace_bytes": ([_vp, _int, _vp], _int),
+    "cupoly_hom_mul_plan_create": ([_vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _int,
+                                    ctypes.POINTER(_vp)], _int),
+    "cupoly_hom_mul_plan_run": ([_vp], _int),
+    "cupoly_hom_mul_plan_destroy": ([_vp], _int),
     "cupoly_polymul_set_tuning": ([_vp, _int, _int], _int),
     "cupoly_polymul_autotune": ([_vp, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "cupoly_sample_muladd_batch": ([_vp, _vp, _vp, _int, _u64, _u64], _int),
@@ -457,6 +461,55 @@ def hom_mul_relin_workspace_bytes(engine, batch):
     out = ctypes.c_uint64()
     _check(lib().cupoly_hom_mul_relin_workspace_bytes(engine.handle, int(batch), ctypes.byref(out)))
     return out.value
+
+
+class HomMulPlan:
+    """A batch of multiplication requests of mixed ring shapes run as one
+    graph launch.  engines: one CUDAEngine per shape; keys: None, or one
+    RelinKey (or None) per shape; requests: list of (shape index, out, ct1,
+    ct2) device tensors, ct [2, k, N, 2] of that shape and out [2, k, N, 2]
+    (relinearized) or [3, k, N, 2]."""
+
+    def __init__(self, ctx, engines, requests, keys=None):
+        self.handle = None
+        cnt = len(requests)
+        for si, o, a, b in requests:
+            if not 0 <= si < len(engines):
+                raise ValueError("cupoly_amd: hom-mul plan request names no shape (%d)" % si)
+            e = engines[si]
+            m = 2 if keys is not None and keys[si] is not None else 3
+            for t, want in ((a, (2, e.k, e.N, 2)), (b, (2, e.k, e.N, 2)), (o, (m, e.k, e.N, 2))):
+                if tuple(t.shape) != want:
+                    raise ValueError("cupoly_amd: hom-mul plan operand shape %s, expected %s"
+                                     % (tuple(t.shape), want))
+        arrp = ctypes.c_void_p * max(cnt, 1)
+        eng = (ctypes.c_void_p * len(engines))(*[e.handle.value for e in engines])
+        ks = None
+        if keys is not None:
+            ks = (ctypes.c_void_p * len(engines))(*[k.handle.value if k is not None else None
+                                                    for k in keys])
+        sh = (ctypes.c_int * max(cnt, 1))(*[r[0] for r in requests])
+        h = ctypes.c_void_p()
+        _check(lib().cupoly_hom_mul_plan_create(
+            ctx.handle, eng, ks, len(engines), sh, arrp(*[_ptr(r[1]).value for r in requests]),
+            arrp(*[_ptr(r[2]).value for r in requests]), arrp(*[_ptr(r[3]).value for r in requests]),
+            cnt, ctypes.byref(h)))
+        self.handle = h
+        self._keep = (engines, keys, requests)
+
+    def run(self):
+        _check(lib().cupoly_hom_mul_plan_run(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().cupoly_hom_mul_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 NOALLOC_SCRATCH_BYTES = 128 << 20

@@ -838,6 +838,37 @@ static size_t hom_mul_relin_ws_bytes(const cupoly_engine *e, int batch) {
   return t + (h > r ? h : r);
 }
 
+// The pipeline for gathered operands given as device pointer tables, with
+// scratch ws of cp_hom_mul_ws_bytes(e, batch, key != nullptr) bytes; key =
+// nullptr: tensor product + rescale into out[i] = [3][k][n/2], else also
+// relinearized into out[i] = [2][k][n/2].  Allocates nothing (a graph may
+// capture it).
+int cp_hom_mul_tables_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out,
+                          cupoly_gi *const *d_ct1, cupoly_gi *const *d_ct2, int batch, void *ws) {
+  cp_cts a{nullptr, 0, (cp_gi *const *)d_ct1}, b{nullptr, 0, (cp_gi *const *)d_ct2};
+  if (!key) {
+    cp_cts o{nullptr, 0, (cp_gi *const *)d_out};
+    return hom_mul_run(c, o, a, b, batch, ws);
+  }
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_gi *T = (cp_gi *)ws;
+  void *shared = T + (size_t)batch * 3 * k * ng;
+  cp_cts t{T, 3 * k * ng, nullptr};
+  int rc = hom_mul_run(c, t, a, b, batch, shared);
+  if (rc) return rc;
+  return cp_hom_mul_relin_run(c, key, d_out, nullptr, T, batch, shared);
+}
+
+// A batch with a (pair, residue) unit for every CU keeps the device busy on
+// its own: running it beside another such batch only shares the CUs.
+bool cp_hom_mul_fills(const cupoly_engine *e, int batch) {
+  return (long)batch * (e->k + e->kl) >= e->ncu;
+}
+
+size_t cp_hom_mul_ws_bytes(const cupoly_engine *e, int batch, bool relin) {
+  return relin ? hom_mul_relin_ws_bytes(e, batch) : hom_mul_ws_bytes(e, batch);
+}
+
 static int hom_mul_relin_alloc_run(cupoly_ctx *c, const cupoly_relin_key *key,
                                    cupoly_gi *const *d_out_table, cupoly_gi *out_base, cp_cts ct1,
                                    cp_cts ct2, int batch) {

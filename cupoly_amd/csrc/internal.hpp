@@ -118,6 +118,11 @@ int cp_relin_setup(cupoly_engine *e);
 // `batch` pointers, or out_base + i * 2 k n/2 when the table is null)
 size_t cp_relin_ws_bytes(const cupoly_engine *e, int batch);
 const cupoly_engine *cp_relin_key_engine(const cupoly_relin_key *key);
+// hom-mul pipeline over device pointer tables (homomul.hip); allocates nothing
+size_t cp_hom_mul_ws_bytes(const cupoly_engine *e, int batch, bool relin);
+bool cp_hom_mul_fills(const cupoly_engine *e, int batch);
+int cp_hom_mul_tables_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out,
+                          cupoly_gi *const *d_ct1, cupoly_gi *const *d_ct2, int batch, void *ws);
 int cp_hom_mul_relin_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out_table,
                          cupoly_gi *out_base, const cp_gi *T, int batch, void *ws);
 int cp_pointwise_mul_launch(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nres);

@@ -28,6 +28,7 @@ typedef struct cupoly_engine cupoly_engine;     /* CUDAEngine + DGTEngine state 
 typedef struct cupoly_ctx cupoly_ctx;           /* Context (stream + scratch)   */
 typedef struct cupoly_uneven cupoly_uneven;     /* plan for an uneven batch      */
 typedef struct cupoly_relin_key cupoly_relin_key; /* resident relinearization key */
+typedef struct cupoly_hom_mul_plan cupoly_hom_mul_plan; /* mixed-shape multiply batch  */
 
 /* Error codes (the reference asserts / throws std::runtime_error / exit(1)). */
 enum {
@@ -289,6 +290,27 @@ int cupoly_hom_mul_relin_gather(cupoly_ctx *c, const cupoly_relin_key *key,
                                 cupoly_gi *const *out, const cupoly_gi *const *ct1,
                                 const cupoly_gi *const *ct2, int batch);
 int cupoly_hom_mul_relin_workspace_bytes(const cupoly_engine *e, int batch, uint64_t *bytes);
+
+/* Mixed-shape multiplication batches (no reference counterpart: the
+ * reference holds one ring shape per process, cudaengine.h:71-93).  count
+ * requests; request i has ring shape shape[i] in [0, n_shapes), the shape of
+ * engines[shape[i]], operands ct1[i], ct2[i] ([2][k][n/2] GIs of that shape)
+ * and output out[i]: [2][k][n/2] when keys[shape[i]] is a relinearization
+ * key of that engine (tensor product, HPS rescale, relinearization), [3][k][n/2]
+ * when keys is NULL or keys[shape[i]] is NULL (tensor product and rescale).
+ * Every shape's pipeline is captured as one branch of a single hipGraph, with
+ * its own context, stream, workspace and device pointer tables allocated
+ * here; the buffers must stay valid for the plan's lifetime, and each run
+ * reads the operands they hold at that time.  Creation synchronises c's
+ * stream. */
+int cupoly_hom_mul_plan_create(cupoly_ctx *c, cupoly_engine *const *engines,
+                               const cupoly_relin_key *const *keys, int n_shapes,
+                               const int *shape, cupoly_gi *const *out,
+                               const cupoly_gi *const *ct1, const cupoly_gi *const *ct2,
+                               int count, cupoly_hom_mul_plan **plan);
+/* One hipGraphLaunch of the plan on c's stream. */
+int cupoly_hom_mul_plan_run(cupoly_hom_mul_plan *plan);
+int cupoly_hom_mul_plan_destroy(cupoly_hom_mul_plan *plan);
 
 /* Encryption-shaped step for a batch: out[i] = a[i] * s_i + e_i in the DGT
  * domain of base Q, s_i ternary (stream stream0 + 2i), e_i discrete Gaussian

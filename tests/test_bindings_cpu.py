@@ -101,6 +101,21 @@ def test_uneven_plan_checks_every_pointer(fake):
         assert rec.calls == [], label
 
 
+def test_hom_mul_plan_checks_every_pointer(fake):
+    rec, ctx, eng = fake
+    eng.handle = types.SimpleNamespace(value=None)
+    k, N = eng.k, eng.N
+    ct = torch.zeros((2, k, N, 2), dtype=torch.int64)
+    for label, bad, msg in _bad_variants((3, k, N, 2)):
+        # outputs are converted first: a bad output alone stops the call
+        with pytest.raises(ValueError, match=msg):
+            cp.HomMulPlan(ctx, [eng], [(0, bad, ct, ct)])
+        assert rec.calls == [], label
+    with pytest.raises(ValueError, match="names no shape"):
+        cp.HomMulPlan(ctx, [eng], [(1, bad, bad, bad)])
+    assert rec.calls == []
+
+
 def test_shape_mismatch_is_refused(fake):
     rec, ctx, eng = fake
     t = torch.zeros((eng.k + 1, eng.N, 2), dtype=torch.int64)
