@@ -541,6 +541,47 @@ inline std::vector<uint64_t> poly_get_integers(Context *ctx, poly_t *a) {
   return out;
 }
 
+// poly_set_coeff / poly_get_coeff (polynomial.h:461-470).  The reference
+// passes NTL ZZ values; here a coefficient is an unsigned integer as
+// little-endian 64-bit limbs.  set: coefficient `index` (0 <= index < n) of a
+// becomes the integer, reduced mod every prime of a's base (the polynomial is
+// returned to the coefficient domain, the residues written, and transformed
+// back, as the reference's host path does).  get: coefficient `index` of a
+// (base Q) in [0, Q), k limbs.
+inline void poly_set_coeff(Context *ctx, poly_t *a, int index, const std::vector<uint64_t> &value) {
+  const int nres = CUDAEngine::get_n_residues(a->base), N = CUDAEngine::N;
+  if (index < 0 || index >= 2 * N) throw std::runtime_error("poly_set_coeff: index out of range");
+  poly_init(ctx, a, a->base);
+  std::vector<uint64_t> primes;  // the primes of a's base, in residue order
+  if (a->base != BBase) primes = CUDAEngine::RNSPrimes;
+  if (a->base != QBase)
+    primes.insert(primes.end(), CUDAEngine::RNSBPrimes.begin(), CUDAEngine::RNSBPrimes.end());
+  std::vector<uint64_t> res(nres);
+  for (int r = 0; r < nres; r++) {
+    const uint64_t p = primes[r];
+    unsigned __int128 acc = 0;  // Horner from the top limb: exact, acc < p
+    for (size_t l = value.size(); l-- > 0;) acc = ((acc << 64) + value[l]) % p;
+    res[r] = (uint64_t)acc;
+  }
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, a->d_coefs, a->base, INVERSE));
+  // coefficient j lives in re of slot j (j < N) or imag of slot j - N
+  uint64_t *dst = index < N ? &a->d_coefs[index].re : &a->d_coefs[index - N].imag;
+  cupoly_detail::SyncOnExit guard{ctx->get_stream()};
+  cupoly_detail::hip_check(hipMemcpy2DAsync(dst, (size_t)N * sizeof(GaussianInteger), res.data(),
+                                            sizeof(uint64_t), sizeof(uint64_t), nres,
+                                            hipMemcpyHostToDevice, ctx->get_stream()),
+                           "poly_set_coeff");
+  cupoly_detail::check(cupoly_dgt(ctx->ctx, a->d_coefs, a->base, FORWARD));
+  a->status = RNSSTATE;
+}
+inline std::vector<uint64_t> poly_get_coeff(Context *ctx, poly_t *a, int index) {
+  const int k = CUDAEngine::get_n_residues(QBase), N = CUDAEngine::N;
+  if (a->base != QBase) throw std::runtime_error("poly_get_coeff: a must be in base Q");
+  if (index < 0 || index >= 2 * N) throw std::runtime_error("poly_get_coeff: index out of range");
+  const std::vector<uint64_t> all = poly_get_integers(ctx, a);
+  return std::vector<uint64_t>(all.begin() + (size_t)index * k, all.begin() + (size_t)(index + 1) * k);
+}
+
 inline bool poly_are_equal(Context *ctx, poly_t *a, poly_t *b) {
   if (a->base != b->base) return false;
   return poly_get_residues(ctx, a) == poly_get_residues(ctx, b);

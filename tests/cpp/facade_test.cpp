@@ -80,6 +80,40 @@ int main(int argc, char **argv) {
           return 1;
         }
       }
+      // poly_set_coeff / poly_get_coeff (the reference's SetCoeff test,
+      // test_bfv_mode.cu:1108-1117): coefficient 3 becomes 42, a coefficient
+      // in the second half becomes a three-limb integer, every other stays
+      {
+        poly_t cc;  // a copy: c stays as it is for the checks below
+        poly_init(&ctx, &cc);
+        poly_copy(&ctx, &cc, &c);
+        const std::vector<uint64_t> before = poly_get_integers(&ctx, &cc);
+        poly_set_coeff(&ctx, &cc, 3, {42});
+        const int hi = CUDAEngine::N + 5;
+        const std::vector<uint64_t> big = {0x0123456789abcdefULL, 0xfedcba9876543210ULL, 7};
+        poly_set_coeff(&ctx, &cc, hi, big);
+        const std::vector<uint64_t> after = poly_get_integers(&ctx, &cc);
+        std::vector<uint64_t> want42(k, 0);
+        want42[0] = 42;
+        bool ok = poly_get_coeff(&ctx, &cc, 3) == want42;
+        // the three-limb value reduced mod Q (k limbs): compare residues instead
+        const std::vector<uint64_t> got_hi = poly_get_coeff(&ctx, &cc, hi);
+        for (int r = 0; r < k && ok; r++) {
+          const uint64_t q = CUDAEngine::RNSPrimes[r];
+          u128 x = 0, y = 0;
+          for (int l = 2; l >= 0; l--) x = ((x << 64) + big[l]) % q;
+          for (int l = k - 1; l >= 0; l--) y = ((y << 64) + got_hi[l]) % q;
+          ok = x == y;
+        }
+        for (int j = 0; j < 2 * CUDAEngine::N && ok; j++)
+          if (j != 3 && j != hi)
+            for (int l = 0; l < k; l++) ok = ok && after[(size_t)j * k + l] == before[(size_t)j * k + l];
+        poly_free(&ctx, &cc);
+        if (!ok) {
+          std::fprintf(stderr, "poly_set_coeff / poly_get_coeff mismatch\n");
+          return 1;
+        }
+      }
       // slot-wise and scalar wrappers against residue arithmetic done here
       // (the DGT is linear, and an integer x is the constant polynomial x)
       {
