@@ -604,7 +604,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_polymul_fused<13,1>", "avg_launch_ms": avg_launch_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     # what actually limits the kernel: 64-bit modular products on
+                     # 32-bit integer lanes (the fraction of that peak is below)
+                     "binding_resource": "integer VALU issue",
+                     "valu_issue_frac": integer["valu_issue_frac"]},
         "integer_issue": integer,
         "cpu_baseline": cpu,
         "hom_mul": hm,

@@ -10,8 +10,12 @@
 //      in every residue) and packs each Gaussian slot's four small values
 //      (s re, s imag, e re, e imag) as signed bytes: 4 bytes per slot;
 //   2. one workgroup per (instance, residue) reads its slots' packed values,
-//      reduces them mod q_r, twists and transforms s and e in LDS, multiplies
-//      by a (read once, in the DGT domain) and writes out.
+//      reduces them mod q_r, twists and transforms e and then s in LDS, and
+//      writes a * s + e slot by slot as it reads a (once, DGT domain).  With e
+//      transformed first, its spectrum is the only value held across the
+//      transform of s, and a is read only at the end: no register spill at
+//      128 VGPRs (the order s, a * s, e held a * s across the transform of e
+//      and spilled 92 bytes per lane).
 // HBM traffic: a in and out out, 16 k n bytes per instance, plus the packed
 // samples (2 n bytes written once, read k times).
 //
@@ -40,15 +44,6 @@ __global__ void __launch_bounds__(256)
 
 __device__ __forceinline__ int se_byte(uint32_t w, int k) { return (int)(int8_t)(w >> (8 * k)); }
 
-// Wave-uniform pointer, kept in SGPRs.
-template <class P>
-__device__ __forceinline__ P *enc_uniform(P *q) {
-  const uint64_t v = (uint64_t)q;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (P *)(((uint64_t)hi << 32) | lo);
-}
-
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_sample_muladd(cp_gi *__restrict__ out, const cp_gi *__restrict__ a, int nunits, int nres,
@@ -73,34 +68,9 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const uint32_t *se = se_all + inst * N;
   const cp_gi *ap = a + (size_t)unit * N;
   cp_gi *op = out + (size_t)unit * N;
-  if (ROWS == 1) {
-    se = enc_uniform(se);
-    ap = enc_uniform(ap);
-    op = enc_uniform(op);
-  }
   MapIdentity id;
   cp_gi x[EPT], y[EPT];
-  // s: ternary, coefficient g in re and g + N in imag of slot g
-  const int ts = cp_opaque(t);
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    const int g = first_index<LOGN, 0, LOGN>(ts, i);
-    const uint32_t w = se[g];
-    x[i].re = small_mod(se_byte(w, 0), p);
-    x[i].imag = small_mod(se_byte(w, 1), p);
-    twist(x[i], tsf, g, p);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
-  // a * s at the spectrum positions this thread holds (last-pass layout),
-  // one slot's load in flight at a time
-  const int ta = cp_opaque(t);
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    x[i] = cp_gimul(ap[pass_index<LOGN, LP::LS0, LP::RL>(ta, i)], x[i], pr);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  // e: discrete Gaussian
+  // e first: its spectrum is the only value held across the transform of s
   const int te = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -112,14 +82,28 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(y, lds, t, twf, p, id);
+  const int ts = cp_opaque(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int g = first_index<LOGN, 0, LOGN>(ts, i);
+    const uint32_t w = se[g];
+    x[i].re = small_mod(se_byte(w, 0), p);
+    x[i].imag = small_mod(se_byte(w, 1), p);
+    twist(x[i], tsf, g, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
   const int to = cp_opaque(t);
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
+      const int j = pass_index<LOGN, LP::LS0, LP::RL>(to, i);
+      const cp_gi m = cp_gimul(ap[j], x[i], pr);
       cp_gi r;
-      r.re = cp_add(x[i].re, y[i].re, p);
-      r.imag = cp_add(x[i].imag, y[i].imag, p);
-      op[pass_index<LOGN, LP::LS0, LP::RL>(to, i)] = r;
+      r.re = cp_add(m.re, y[i].re, p);
+      r.imag = cp_add(m.imag, y[i].imag, p);
+      op[j] = r;
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
