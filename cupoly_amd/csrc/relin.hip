@@ -190,7 +190,12 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  cp_gi *o = out.at(pair);
+  // the output address may come from a pointer table: typed as global memory,
+  // so its stores are global stores (a generic pointer's flat stores also
+  // count against the LDS wait counter of the exchanges)
+  typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(1))) u64x2 global_gi;
+  global_gi *o = (global_gi *)out.at(pair);
 #pragma unroll
   for (int m = 0; m < 2; m++) {
     if (m == 1) {
@@ -202,13 +207,13 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     const int ts = cp_opaque(t);
     if (active) {
       const cp_gi *ym = y + ((size_t)m * k + j) * N;
-      cp_gi *om = o + ((size_t)m * k + j) * N;
+      global_gi *om = o + ((size_t)m * k + j) * N;
 #pragma unroll
       for (int r = 0; r < EPT; r++) {
         const int g = first_index<LOGN, 0, LOGN>(ts, r);
         twist(acc[r], tsi, g, p);
         const cp_gi v = ym[g];
-        om[g] = cp_gi{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)};
+        om[g] = u64x2{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)};
         __builtin_amdgcn_sched_barrier(0);
       }
     }
