@@ -769,9 +769,10 @@ static size_t hom_mul_ws_form(const cupoly_engine *e, int batch, bool fti_form) 
   return (size_t)batch * (4 * xrows + 3 * (size_t)(e->k + e->kl)) * e->ng * sizeof(cp_gi);
 }
 
-// Scratch that covers every batch of up to `batch` pairs (the two forms need
-// different amounts per pair, so the figure is the larger over both ranges).
-static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
+// Scratch that covers every batch of up to `batch` pairs run as one pipeline
+// (the two forms need different amounts per pair, so the figure is the
+// larger over both ranges).
+static size_t hom_mul_ws_single(const cupoly_engine *e, int batch) {
   if (!hom_mul_fti_sizes(e)) return hom_mul_ws_form(e, batch, false);
   const int b0 = hom_mul_fti_min_batch(e);
   const size_t f = batch >= b0 ? hom_mul_ws_form(e, batch, true) : 0;
@@ -780,7 +781,7 @@ static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
 }
 
 // The pipeline on the context's stream with scratch `ws` of at least
-// hom_mul_ws_bytes(e, batch) bytes; allocates nothing.
+// hom_mul_ws_single(e, batch) bytes; allocates nothing.
 static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch, void *ws) {
   cupoly_engine *e = c->e;
   const int k = e->k, kl = e->kl, nres = k + kl, ng = e->ng;
@@ -837,10 +838,90 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
 
 // The full multiplication (relin.hip relinearizes): the rescaled tensor
 // T = [batch][3][k][n/2] words, then the scratch shared by the two steps.
-static size_t hom_mul_relin_ws_bytes(const cupoly_engine *e, int batch) {
+static size_t hom_mul_relin_ws_single(const cupoly_engine *e, int batch) {
   const size_t t = (size_t)batch * 3 * e->k * e->ng * sizeof(cp_gi);
-  const size_t h = hom_mul_ws_bytes(e, batch), r = cp_relin_ws_bytes(e, batch);
+  const size_t h = hom_mul_ws_single(e, batch), r = cp_relin_ws_bytes(e, batch);
   return t + (h > r ? h : r);
+}
+
+// A batch whose halves each still have a (pair, residue) unit per CU runs as
+// two pipelines on the context's two streams.  With one stream the last
+// round of the one-workgroup-per-unit kernel leaves CUs idle (1152 units at
+// n = 2^14, k = 4, 128 pairs are 4.5 rounds over 256 CUs) and the HPS steps
+// wait for it; with two, one half's extension, scaling and relinearization
+// fill the CUs the other half's transforms leave free (measured: 128 pairs
+// 1.63 against 1.73 ms, 2.20 against 2.36 ms relinearized).
+static bool hom_mul_splits(const cupoly_engine *e, int batch) {
+  return hom_mul_fti_sizes(e) && batch >= 2 * hom_mul_fti_min_batch(e);
+}
+
+static size_t pipeline_ws(const cupoly_engine *e, int batch, bool relin) {
+  return relin ? hom_mul_relin_ws_single(e, batch) : hom_mul_ws_single(e, batch);
+}
+
+// Scratch for every batch of up to `batch` pairs, split or not: the two
+// halves' scratch areas lie one after the other.
+static size_t hom_mul_cover_ws(const cupoly_engine *e, int batch, bool relin) {
+  size_t w = pipeline_ws(e, batch, relin);
+  if (hom_mul_splits(e, batch)) {
+    const int a = (batch + 1) / 2;
+    const size_t s = pipeline_ws(e, a, relin) + pipeline_ws(e, batch - a, relin);
+    if (s > w) w = s;
+  }
+  return w;
+}
+static size_t hom_mul_ws_bytes(const cupoly_engine *e, int batch) {
+  return hom_mul_cover_ws(e, batch, false);
+}
+static size_t hom_mul_relin_ws_bytes(const cupoly_engine *e, int batch) {
+  return hom_mul_cover_ws(e, batch, true);
+}
+
+// Request i of a batch onwards.
+static cp_cts cts_from(const cp_cts &x, int i) {
+  return x.table ? cp_cts{nullptr, 0, x.table + i} : cp_cts{x.base + (size_t)i * x.stride, x.stride, nullptr};
+}
+
+// One pipeline on c's stream with scratch ws of pipeline_ws(e, batch, key):
+// tensor product + rescale into out ([3][k][n/2] per pair), or with a key
+// also relinearized (out [2][k][n/2] per pair; T and the scratch shared by
+// the two steps in ws).
+static int hom_mul_one(cupoly_ctx *c, const cupoly_relin_key *key, cp_cts out, cp_cts ct1,
+                       cp_cts ct2, int batch, void *ws) {
+  if (!key) return hom_mul_run(c, out, ct1, ct2, batch, ws);
+  const size_t k = c->e->k, ng = c->e->ng;
+  cp_gi *T = (cp_gi *)ws;
+  void *shared = T + (size_t)batch * 3 * k * ng;
+  cp_cts t{T, 3 * k * ng, nullptr};
+  int rc = hom_mul_run(c, t, ct1, ct2, batch, shared);
+  if (rc) return rc;
+  return cp_hom_mul_relin_run(c, key, (cupoly_gi *const *)out.table, (cupoly_gi *)out.base, T,
+                              batch, shared);
+}
+
+// The whole batch with scratch ws of hom_mul_cover_ws(e, batch, key):
+// allocates nothing (a graph may capture it: the second stream joins the
+// capture through the fork event and is joined back before the return).  The
+// halves go to two streams only when the context already has its second
+// stream (the allocating entry points make it); otherwise one pipeline runs.
+static int hom_mul_pipeline(cupoly_ctx *c, const cupoly_relin_key *key, cp_cts out, cp_cts ct1,
+                            cp_cts ct2, int batch, void *ws) {
+  if (!c->aux || !hom_mul_splits(c->e, batch)) return hom_mul_one(c, key, out, ct1, ct2, batch, ws);
+  const int a = (batch + 1) / 2, b = batch - a;
+  cupoly_ctx *x = c->aux;
+  x->wide_slots = c->wide_slots;
+  x->scale_chain = c->scale_chain;
+  CP_HIP(hipEventRecord(c->ev_fork, c->stream));
+  CP_HIP(hipStreamWaitEvent(x->stream, c->ev_fork, 0));
+  int rc = hom_mul_one(c, key, out, ct1, ct2, a, ws);
+  if (!rc)
+    rc = hom_mul_one(x, key, cts_from(out, a), cts_from(ct1, a), cts_from(ct2, a), b,
+                     (char *)ws + pipeline_ws(c->e, a, key != nullptr));
+  // joined back also after a failed launch, so the stream state stays whole
+  const hipError_t e1 = hipEventRecord(c->ev_join, x->stream);
+  const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(c->stream, c->ev_join, 0) : e1;
+  if (rc) return rc;
+  return e2 == hipSuccess ? CUPOLY_OK : cp_check_hip(e2, "hom_mul: join");
 }
 
 // The pipeline for gathered operands given as device pointer tables, with
@@ -850,24 +931,21 @@ static size_t hom_mul_relin_ws_bytes(const cupoly_engine *e, int batch) {
 // capture it).
 int cp_hom_mul_tables_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out,
                           cupoly_gi *const *d_ct1, cupoly_gi *const *d_ct2, int batch, void *ws) {
-  cp_cts a{nullptr, 0, (cp_gi *const *)d_ct1}, b{nullptr, 0, (cp_gi *const *)d_ct2};
-  if (!key) {
-    cp_cts o{nullptr, 0, (cp_gi *const *)d_out};
-    return hom_mul_run(c, o, a, b, batch, ws);
-  }
-  const size_t k = c->e->k, ng = c->e->ng;
-  cp_gi *T = (cp_gi *)ws;
-  void *shared = T + (size_t)batch * 3 * k * ng;
-  cp_cts t{T, 3 * k * ng, nullptr};
-  int rc = hom_mul_run(c, t, a, b, batch, shared);
-  if (rc) return rc;
-  return cp_hom_mul_relin_run(c, key, d_out, nullptr, T, batch, shared);
+  cp_cts o{nullptr, 0, (cp_gi *const *)d_out}, a{nullptr, 0, (cp_gi *const *)d_ct1},
+      b{nullptr, 0, (cp_gi *const *)d_ct2};
+  return hom_mul_pipeline(c, key, o, a, b, batch, ws);
 }
 
 // A batch with a (pair, residue) unit for every CU keeps the device busy on
 // its own: running it beside another such batch only shares the CUs.
 bool cp_hom_mul_fills(const cupoly_engine *e, int batch) {
   return (long)batch * (e->k + e->kl) >= e->ncu;
+}
+
+// Everything a captured cp_hom_mul_tables_run of `batch` pairs needs besides
+// its scratch: the second stream, when the batch splits.
+int cp_hom_mul_prepare(cupoly_ctx *c, int batch) {
+  return hom_mul_splits(c->e, batch) ? cp_ctx_aux(c) : CUPOLY_OK;
 }
 
 size_t cp_hom_mul_ws_bytes(const cupoly_engine *e, int batch, bool relin) {
@@ -880,21 +958,20 @@ static int hom_mul_relin_alloc_run(cupoly_ctx *c, const cupoly_relin_key *key,
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   void *ws;
   int rc = cp_workspace(c, hom_mul_relin_ws_bytes(c->e, batch), &ws);
+  if (!rc && hom_mul_splits(c->e, batch)) rc = cp_ctx_aux(c);
   if (rc) return rc;
   const size_t k = c->e->k, ng = c->e->ng;
-  cp_gi *T = (cp_gi *)ws;
-  void *shared = T + (size_t)batch * 3 * k * ng;
-  cp_cts t{T, 3 * k * ng, nullptr};
-  if ((rc = hom_mul_run(c, t, ct1, ct2, batch, shared))) return rc;
-  return cp_hom_mul_relin_run(c, key, d_out_table, out_base, T, batch, shared);
+  cp_cts o{(cp_gi *)out_base, 2 * k * ng, (cp_gi *const *)d_out_table};
+  return hom_mul_pipeline(c, key, o, ct1, ct2, batch, ws);
 }
 
 static int hom_mul_alloc_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int batch) {
   if (batch > 65535 / 4) return cp_fail(CUPOLY_ERR_ARG, "hom_mul: batch > 16383");
   void *ws;
   int rc = cp_workspace(c, hom_mul_ws_bytes(c->e, batch), &ws);
+  if (!rc && hom_mul_splits(c->e, batch)) rc = cp_ctx_aux(c);
   if (rc) return rc;
-  return hom_mul_run(c, out, ct1, ct2, batch, ws);
+  return hom_mul_pipeline(c, nullptr, out, ct1, ct2, batch, ws);
 }
 
 extern "C" int cupoly_hom_mul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *ct1,
@@ -984,6 +1061,7 @@ extern "C" int cupoly_hom_mul_reserve(cupoly_ctx *c, int batch) {
   if (batch == 0) return CUPOLY_OK;
   void *ws;  // room for the relinearizing form, the larger of the two
   int rc = cp_workspace(c, hom_mul_relin_ws_bytes(c->e, batch), &ws);
+  if (!rc && hom_mul_splits(c->e, batch)) rc = cp_ctx_aux(c);
   if (rc) return rc;
   return grow_ptr_tables(c, (size_t)batch * 3 * sizeof(void *));
 }
@@ -1028,5 +1106,5 @@ extern "C" int cupoly_hom_mul_batch_noalloc(cupoly_ctx *c, cupoly_gi *out, const
   const size_t k = c->e->k, ng = c->e->ng;
   cp_cts o{(cp_gi *)out, 3 * k * ng, nullptr}, a{(cp_gi *)ct1, 2 * k * ng, nullptr},
       b{(cp_gi *)ct2, 2 * k * ng, nullptr};
-  return hom_mul_run(c, o, a, b, batch, ws);
+  return hom_mul_pipeline(c, nullptr, o, a, b, batch, ws);
 }

@@ -97,6 +97,7 @@ extern "C" int cupoly_hom_mul_plan_create(cupoly_ctx *c, cupoly_engine *const *e
     p->groups.push_back(g);
     auto &G = p->groups.back();
     if ((rc = cupoly_ctx_create(engines[s], nullptr, &G.ctx))) break;
+    if ((rc = cp_hom_mul_prepare(G.ctx, G.count))) break;
     const size_t ws = cp_hom_mul_ws_bytes(engines[s], G.count, G.key != nullptr);
     std::vector<cupoly_gi *> h(3 * (size_t)G.count);
     for (int j = 0; j < G.count; j++) {

@@ -90,7 +90,15 @@ struct cupoly_ctx {
   // hom-mul t/Q scaling: the reference's addmod chain on every slot instead of
   // the exact sums (which give the same words)
   bool scale_chain = false;
+  // a second stream (default priority, bufferless context) and the events
+  // that fork it from `stream` and join it back: a hom-mul batch that fills
+  // the CUs twice over runs its two halves on the two streams, so one half's
+  // HPS steps and the other's transforms share the CUs.  Made on first need
+  // (cp_ctx_aux), by the calls that may allocate.
+  cupoly_ctx *aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
+int cp_ctx_aux(cupoly_ctx *c);
 
 // error plumbing
 int cp_fail(int code, const std::string &msg);
@@ -121,6 +129,7 @@ const cupoly_engine *cp_relin_key_engine(const cupoly_relin_key *key);
 // hom-mul pipeline over device pointer tables (homomul.hip); allocates nothing
 size_t cp_hom_mul_ws_bytes(const cupoly_engine *e, int batch, bool relin);
 bool cp_hom_mul_fills(const cupoly_engine *e, int batch);
+int cp_hom_mul_prepare(cupoly_ctx *c, int batch);
 int cp_hom_mul_tables_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out,
                           cupoly_gi *const *d_ct1, cupoly_gi *const *d_ct2, int batch, void *ws);
 int cp_hom_mul_relin_run(cupoly_ctx *c, const cupoly_relin_key *key, cupoly_gi *const *d_out_table,

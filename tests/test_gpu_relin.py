@@ -138,3 +138,36 @@ def test_relinearize_fused_and_composed_forms_agree(gpu, k, n, batch):
     h = cp.to_host(full)
     for i in (0, batch - 1):
         assert np.array_equal(h[i], O.relinearize(P, y[i], key)), i
+
+
+@pytest.mark.parametrize("k,n,batch", [(3, 8192, 75), (4, 16384, 58), (2, 2048, 104)])
+def test_two_stream_halves_equal_single_pipelines(gpu, k, n, batch):
+    """A batch whose halves each have a (pair, residue) unit per CU runs its
+    halves on the context's two streams; every pair equals the same pairs run
+    as two smaller batches (one pipeline each), in the contiguous, gathered
+    and relinearizing forms, and one pair equals the oracle composition."""
+    import torch
+    e, ctx, P, s_vals, rk = setup(k, n)
+    rng = np.random.default_rng(13 * n + batch)
+    ct = np.empty((2, batch, 2, k, P.Ng, 2), dtype=np.uint64)
+    for j, qj in enumerate(P.q):
+        ct[:, :, :, j] = rng.integers(0, qj, size=(2, batch, 2, P.Ng, 2), dtype=np.uint64)
+    c1, c2 = cp.to_device(ct[0]), cp.to_device(ct[1])
+    a = (batch + 1) // 2
+    full = cp.empty((batch, 3, k, P.Ng, 2))
+    cp.hom_mul_batch(ctx, full, c1, c2)
+    parts = cp.empty((batch, 3, k, P.Ng, 2))
+    cp.hom_mul_batch(ctx, parts[:a], c1[:a], c2[:a])
+    cp.hom_mul_batch(ctx, parts[a:], c1[a:], c2[a:])
+    assert torch.equal(full, parts)
+    gathered = [cp.empty((3, k, P.Ng, 2)) for _ in range(batch)]
+    cp.hom_mul_gather(ctx, gathered, [c1[i] for i in range(batch)], [c2[i] for i in range(batch)])
+    assert torch.equal(torch.stack(gathered), full)
+    rfull = cp.empty((batch, 2, k, P.Ng, 2))
+    cp.hom_mul_relin_batch(ctx, rk, rfull, c1, c2)
+    rparts = cp.empty((batch, 2, k, P.Ng, 2))
+    cp.hom_mul_relin_batch(ctx, rk, rparts[:a], c1[:a], c2[:a])
+    cp.hom_mul_relin_batch(ctx, rk, rparts[a:], c1[a:], c2[a:])
+    assert torch.equal(rfull, rparts)
+    i = batch - 1  # in the second half
+    assert np.array_equal(cp.to_host(full[i]), O.hom_mul(P, ct[0, i], ct[1, i]))
