@@ -102,8 +102,10 @@ __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, TW __restrict__
           const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
           uint64_t w, wp;
           ld_tw(tw, e, w, wp);
-          B.re = cp_shoup(dr, w, wp, p);
-          B.imag = cp_shoup(di, w, wp, p);
+          if (static_j)  // a uniform twiddle (may sit in SGPRs)
+            cp_shoup2(dr, w, wp, di, w, wp, p, B.re, B.imag);
+          else
+            cp_shoup2v(dr, w, wp, di, w, wp, p, B.re, B.imag);
         }
       }
       if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
@@ -140,8 +142,10 @@ __device__ __forceinline__ void dit_pass(cp_gi (&x)[EPT], int t, TW __restrict__
           const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
           uint64_t w, wp;
           ld_tw(tw, e, w, wp);
-          tr = cp_shoup(B.re, w, wp, p);
-          ti = cp_shoup(B.imag, w, wp, p);
+          if (static_j)
+            cp_shoup2(B.re, w, wp, B.imag, w, wp, p, tr, ti);
+          else
+            cp_shoup2v(B.re, w, wp, B.imag, w, wp, p, tr, ti);
         }
         B.re = cp_red2(A.re + p - tr, p);
         B.imag = cp_red2(A.imag + p - ti, p);
@@ -266,8 +270,7 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
                              : (unsigned)((map(base + (q << lst)) & mmask) << s);
       uint64_t w, wp;
       ld_tw(tw, e, w, wp);
-      B.re = cp_shoup(dr, w, wp, p);
-      B.imag = cp_shoup(di, w, wp, p);
+      cp_shoup2v(dr, w, wp, di, w, wp, p, B.re, B.imag);
     }
     // one stage's twiddles in flight at a time (bounds VGPR use)
     if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
@@ -296,8 +299,8 @@ __device__ __forceinline__ void dit_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
                              : (unsigned)((map(base + (q << lst)) & mmask) << s);
       uint64_t w, wp;
       ld_tw(tw, e, w, wp);
-      const uint64_t tr = cp_shoup(B.re, w, wp, p);
-      const uint64_t ti = cp_shoup(B.imag, w, wp, p);
+      uint64_t tr, ti;
+      cp_shoup2v(B.re, w, wp, B.imag, w, wp, p, tr, ti);
       B.re = cp_red2(A.re + p - tr, p);
       B.imag = cp_red2(A.imag + p - ti, p);
       A.re = cp_red2(A.re + tr, p);
@@ -513,8 +516,7 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
     ld_tw(twf, (unsigned)g, w, wp);
     x0[i].re = cp_red2(x.re + y.re, p);
     x0[i].imag = cp_red2(x.imag + y.imag, p);
-    x1[i].re = cp_shoup(x.re + p - y.re, w, wp, p);
-    x1[i].imag = cp_shoup(x.imag + p - y.imag, w, wp, p);
+    cp_shoup2v(x.re + p - y.re, w, wp, x.imag + p - y.imag, w, wp, p, x1[i].re, x1[i].imag);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -531,7 +533,8 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
     const int g = first_index<13, 0, 13>(t, i);
     uint64_t w, wp;
     ld_tw(twi, (unsigned)g, w, wp);
-    const uint64_t tr = cp_shoup(y[i].re, w, wp, p), ti = cp_shoup(y[i].imag, w, wp, p);
+    uint64_t tr, ti;
+    cp_shoup2v(y[i].re, w, wp, y[i].imag, w, wp, p, tr, ti);
     cp_gi s, dd;
     s.re = cp_red2(x[i].re + tr, p);
     s.imag = cp_red2(x[i].imag + ti, p);

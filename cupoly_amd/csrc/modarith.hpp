@@ -35,6 +35,66 @@ CP_HD uint64_t cp_mulhi(uint64_t a, uint64_t b) {
   return (uint64_t)(((cp_u128)a * b) >> 64);
 }
 
+// High words of two 64 x 64 products, hx = (x bx) >> 64 and hy = (y by) >> 64.
+// The middle column of the schoolbook product, x1 bx0 + x0 bx1 + hi(x0 bx0),
+// can reach 2^65.  Left to itself the compiler keeps it below 2^64 by
+// splitting it into zero-extended 32-bit halves: four register moves and a
+// 64-bit add per product.  Two cheaper forms:
+//   * cp_mulhi2: the middle column as a 128-bit sum, which the compiler turns
+//     into an add with carry into a zero word (seven instructions, no move);
+//     valid for operands in SGPRs (uniform twiddles);
+//   * cp_mulhi2v: the second multiply-add of the column takes the first's
+//     full 64-bit result, and its carry-out (an SGPR lane mask) goes into the
+//     top word with one add-with-carry: six instructions per product.  Its asm
+//     takes every operand in VGPRs.  The two products share each asm block,
+//     so at least two other VALU instructions separate a carry's write from
+//     its read (gfx950 needs two wait states between a VALU write of a lane
+//     mask and a VALU read of it as a carry).
+CP_HD uint64_t cp_mulhi_col(uint64_t x, uint64_t b) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t m = (uint64_t)x0 * b1 + (uint64_t)(((uint64_t)x0 * b0) >> 32);  // < 2^64
+  const cp_u128 s = (cp_u128)((uint64_t)x1 * b0) + m;                         // < 2^65
+  return (uint64_t)x1 * b1 + (uint64_t)(s >> 32);
+}
+CP_HD void cp_mulhi2(uint64_t x, uint64_t bx, uint64_t y, uint64_t by, uint64_t &hx,
+                     uint64_t &hy) {
+  hx = cp_mulhi_col(x, bx);
+  hy = cp_mulhi_col(y, by);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void cp_mulhi2v(uint64_t x, uint64_t bx, uint64_t y, uint64_t by,
+                                           uint64_t &hx, uint64_t &hy) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+  const uint32_t bx0 = (uint32_t)bx, bx1 = (uint32_t)(bx >> 32);
+  const uint32_t by0 = (uint32_t)by, by1 = (uint32_t)(by >> 32);
+  // x0 bx1 + hi(x0 bx0) < 2^64
+  const uint64_t mx = (uint64_t)x0 * bx1 + __umulhi(x0, bx0);
+  const uint64_t my = (uint64_t)y0 * by1 + __umulhi(y0, by0);
+  uint64_t sx, sy, cx, cy;  // sx + cx 2^64 = x1 bx0 + mx
+  asm("v_mad_u64_u32 %0, %2, %4, %5, %6\n\t"
+      "v_mad_u64_u32 %1, %3, %7, %8, %9"
+      : "=&v"(sx), "=&v"(sy), "=&s"(cx), "=&s"(cy)
+      : "v"(x1), "v"(bx0), "v"(mx), "v"(y1), "v"(by0), "v"(my));
+  const uint64_t tx = (uint64_t)x1 * bx1 + (sx >> 32);  // < 2^64 - 2^32
+  const uint64_t ty = (uint64_t)y1 * by1 + (sy >> 32);
+  uint32_t ux, uy;
+  uint64_t jx, jy;
+  asm("v_addc_co_u32_e64 %0, %2, %4, 0, %5\n\t"
+      "v_addc_co_u32_e64 %1, %3, %6, 0, %7"
+      : "=&v"(ux), "=&v"(uy), "=&s"(jx), "=&s"(jy)
+      : "v"((uint32_t)(tx >> 32)), "s"(cx), "v"((uint32_t)(ty >> 32)), "s"(cy));
+  hx = ((uint64_t)ux << 32) | (uint32_t)tx;
+  hy = ((uint64_t)uy << 32) | (uint32_t)ty;
+}
+#else
+static inline void cp_mulhi2v(uint64_t x, uint64_t bx, uint64_t y, uint64_t by, uint64_t &hx,
+                              uint64_t &hy) {
+  cp_mulhi2(x, bx, y, by, hx, hy);
+}
+#endif
+
 CP_HD uint64_t cp_add(uint64_t a, uint64_t b, uint64_t p) {
   uint64_t s = a + b;  // a, b < p < 2^63: no wrap
   return s >= p ? s - p : s;
@@ -64,8 +124,7 @@ CP_HD uint64_t cp_red2(uint64_t a, uint64_t p) {
 // p = 2^63 - delta, delta < 2^30 (every prime of the 63-bit bucket), that is
 // a w + q delta + (q mod 2) 2^63: two 32x32+64 multiply-adds for the low word
 // and three 32-bit products for the high word, and no 64-bit subtraction.
-CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
-  const uint64_t q = cp_mulhi(a, wp);
+CP_HD uint64_t cp_shoup_rem(uint64_t a, uint64_t w, uint64_t q, uint64_t p) {
   const uint32_t delta = (uint32_t)((1ULL << 63) - p);
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
   const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
@@ -75,8 +134,30 @@ CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
   return ((uint64_t)hi << 32) | (uint32_t)x;
 }
 
+CP_HD uint64_t cp_shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
+  return cp_shoup_rem(a, w, cp_mulhi(a, wp), p);
+}
+
 CP_HD uint64_t cp_shoup(uint64_t a, uint64_t w, uint64_t wp, uint64_t p) {
   return cp_red2(cp_shoup_lazy(a, w, wp, p), p);
+}
+
+// Two Shoup products, x w_x mod p and y w_y mod p in [0, p).  cp_shoup2v
+// forms the quotients with cp_mulhi2v (every operand in VGPRs: twiddles and
+// twists read per lane); cp_shoup2 with cp_mulhi2 (any operands).
+CP_HD void cp_shoup2(uint64_t x, uint64_t wx, uint64_t wxp, uint64_t y, uint64_t wy,
+                     uint64_t wyp, uint64_t p, uint64_t &rx, uint64_t &ry) {
+  uint64_t qx, qy;
+  cp_mulhi2(x, wxp, y, wyp, qx, qy);
+  rx = cp_red2(cp_shoup_rem(x, wx, qx, p), p);
+  ry = cp_red2(cp_shoup_rem(y, wy, qy, p), p);
+}
+CP_HD void cp_shoup2v(uint64_t x, uint64_t wx, uint64_t wxp, uint64_t y, uint64_t wy,
+                      uint64_t wyp, uint64_t p, uint64_t &rx, uint64_t &ry) {
+  uint64_t qx, qy;
+  cp_mulhi2v(x, wxp, y, wyp, qx, qy);
+  rx = cp_red2(cp_shoup_rem(x, wx, qx, p), p);
+  ry = cp_red2(cp_shoup_rem(y, wy, qy, p), p);
 }
 
 // a * b mod p for a * b < 2^127 (a < 2^64, b < 2^63 or the reverse), with
@@ -160,8 +241,8 @@ CP_HD cp_gi cp_gimul(cp_gi a, cp_gi b, const cp_prime& pr) {
 // Gaussian product by a constant (c + i d) given as three Shoup pairs:
 // k[0..1] = (c, c'), k[2..3] = (d, d'), k[4..5] = (c+d mod p, (c+d)').
 CP_HD cp_gi cp_gimul_const(cp_gi a, const uint64_t* k, uint64_t p) {
-  uint64_t s1 = cp_shoup(a.re, k[0], k[1], p);
-  uint64_t s2 = cp_shoup(a.imag, k[2], k[3], p);
+  uint64_t s1, s2;
+  cp_shoup2v(a.re, k[0], k[1], a.imag, k[2], k[3], p, s1, s2);
   uint64_t s3 = cp_shoup(a.re + a.imag, k[4], k[5], p);  // a.re + a.imag < 2^64
   cp_gi c;
   c.re = cp_sub(s1, s2, p);
