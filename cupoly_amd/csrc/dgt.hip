@@ -209,8 +209,11 @@ __global__ void __launch_bounds__((1 << LOGM_) / EPT)
 }
 
 // -------------------------------------------------------------------------
-// Fused coefficient-domain product c = a*b, N <= 8192: forward(a) and
-// forward(b) in one workgroup, slot-wise Gaussian product, inverse.
+// Fused coefficient-domain product c = a*b, N <= 8192, in the split basis
+// (dgt_dev.hpp): conversion and twist of a, forward network, the same for b,
+// two products mod p per slot, the split-tree inverse and the conversion back,
+// in one workgroup.  The DGT-domain values of a and b never leave it, so the
+// basis does not show; the coefficients are the exact negacyclic product.
 // For N = 4096 (512 threads, 72 KiB of LDS) two workgroups fit on a CU only
 // if each wave keeps to 128 VGPRs, so that size asks for 4 waves per SIMD
 // (fused_min_waves).
@@ -221,8 +224,8 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     k_polymul_fused(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
                     const cp_gi *__restrict__ b, int nunits, int nres,
                     const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                    const uint64_t *__restrict__ twi_all, const uint64_t *__restrict__ tsf_all,
-                    const uint64_t *__restrict__ tsi_all) {
+                    const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
+                    const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   // with one row per workgroup, keep the row (hence the residue and every
@@ -237,20 +240,24 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   if (ROWS == 1) rid = __builtin_amdgcn_readfirstlane(rid);  // uniform: keep in SGPRs
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
-  const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
-  const uint64_t *tsf = tsf_all + (size_t)rid * N * 6, *tsi = tsi_all + (size_t)rid * N * 6;
+  const uint64_t *twf = twf_all + (size_t)rid * N;
+  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4);
+  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const uint64_t *sk = sk_all + (size_t)rid * 8;
+  const uint64_t J = sk[0], Jp = sk[1];
   const size_t off = (size_t)unit * N;
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
-  // a: load, twist, transform; b is loaded only afterwards so that a single
-  // operand is live during each forward transform.  The scheduling barriers
-  // keep the twist-table loads of one slot from being hoisted above the
-  // previous slot's products (which would spill at 128 VGPRs).
+  // a: load, convert and twist, transform; b is loaded only afterwards so
+  // that a single operand is live during each forward transform.  The
+  // scheduling barriers keep the twist-table loads of one slot pair from being
+  // hoisted above the previous pair's products (which would spill).
 #pragma unroll
   for (int i = 0; i < EPT; i++) xa[i] = a[off + first_index<LOGN, 0, LOGN>(t, i)];
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    twist(xa[i], tsf, first_index<LOGN, 0, LOGN>(t, i), p);
+  for (int i = 0; i < EPT; i += 2) {
+    split_twist2(xa[i], xa[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
+                 first_index<LOGN, 0, LOGN>(t, i + 1), J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
@@ -262,21 +269,23 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 #pragma unroll
   for (int i = 0; i < EPT; i++) xb[i] = b[off + first_index<LOGN, 0, LOGN>(tb, i)];
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    twist(xb[i], tsf, first_index<LOGN, 0, LOGN>(tb, i), p);
+  for (int i = 0; i < EPT; i += 2) {
+    split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
+                 first_index<LOGN, 0, LOGN>(tb, i + 1), J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = cp_gimul(xa[i], xb[i], pr);
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twi, p, id);
+  for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
   int to = t;
   asm volatile("" : "+v"(to));
+  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(to, i);
-      twist(xa[i], tsi, g, p);
+      split_back(xa[i], h, hp, hj, hjp, p);
       out[off + g] = xa[i];
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -489,8 +498,8 @@ static void launch_fused(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi 
   cupoly_engine *e = c->e;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
-                     c->stream, out, a, b, nunits, e->k, e->d_primes, e->d_tw_fwd, e->d_tw_inv,
-                     e->d_twist_fwd, e->d_twist_inv);
+                     c->stream, out, a, b, nunits, e->k, e->d_primes, e->d_tw_fwd,
+                     e->d_split_root, e->d_split_tw, e->d_split_k);
 }
 
 // Rows (residue polynomials) per workgroup the batched product can use at

@@ -464,6 +464,187 @@ __device__ __forceinline__ void twist(cp_gi &v, const CP_GLOBAL uint64_t *__rest
 }
 
 
+// ---- Split-basis products --------------------------------------------------
+// omega of order N >= 16 lies in Z_p, so p = 1 (mod 4): -1 has the square
+// roots +-J in Z_p and Z_p[i] = Z_p x Z_p through a + i b -> (a + J b, a - J b).
+// A product whose inputs and outputs are coefficients never exposes the DGT
+// domain, so it may run in that basis.  Slot g = (a_g, a_{g+N}) becomes
+// (u_g, v_g) = (a_g + J a_{g+N}, a_g - J a_{g+N}): the residues of a mod
+// X^N - J and mod X^N + J.  Then
+//   * the twist by xi^g (xi^N = i) is two real products, by xi+^g and xi-^g
+//     (xi+- = xi.re +- J xi.im, so xi+^N = J and xi-^N = -J), in place of a
+//     Gaussian product;
+//   * the forward network (dif_passes, the same real omega) is unchanged and
+//     leaves u(xi+ omega^k) and v(xi- omega^k) at position bitrev(k);
+//   * the slot-wise product is two products mod p instead of a Gaussian one;
+//   * the inverse undoes the split tree of X^N -+ J with Gentleman-Sande
+//     butterflies, A + B and (A - B) r^-1, r the root that split the
+//     butterfly's group (X^2m - r^2 = (X^m - r)(X^m + r)).  That absorbs the
+//     untwist, and the difference feeds a Shoup product unreduced, where the
+//     inverse DGT's butterflies reduce three times;
+//   * back to coefficients, c_g = (U + V) / 2N and c_{g+N} = (U - V) / 2JN,
+//     two Shoup products that also take their inputs unreduced.
+// Tables per residue: twist [N][4] = (xi+^g, Shoup, xi-^g, Shoup); inverse
+// roots [N][4] at node 2^s + group (stage s) = (r_u^-1, Shoup, r_v^-1, Shoup);
+// constants [8] = (J, 1/2N, 1/2JN) as Shoup pairs.
+
+// Forward conversion and twist of slots x (index gx) and y (index gy).
+__device__ __forceinline__ void split_twist2(cp_gi &x, cp_gi &y, const CP_GLOBAL uint64_t *tab,
+                                             int gx, int gy, uint64_t J, uint64_t Jp,
+                                             uint64_t p) {
+  uint64_t tx, ty;
+  cp_shoup2(x.imag, J, Jp, y.imag, J, Jp, p, tx, ty);  // J may sit in SGPRs
+  const CP_GLOBAL cp_u64x2 *kx = (const CP_GLOBAL cp_u64x2 *)(tab + 4 * (size_t)gx);
+  const CP_GLOBAL cp_u64x2 *ky = (const CP_GLOBAL cp_u64x2 *)(tab + 4 * (size_t)gy);
+  const cp_u64x2 px = kx[0], mx = kx[1], py = ky[0], my = ky[1];
+  // a + t and a + p - t are below 2p: Shoup inputs as they are
+  cp_shoup2v(x.re + tx, px.x, px.y, x.re + p - tx, mx.x, mx.y, p, x.re, x.imag);
+  cp_shoup2v(y.re + ty, py.x, py.y, y.re + p - ty, my.x, my.y, p, y.re, y.imag);
+}
+// Same for one slot.
+__device__ __forceinline__ void split_twist1(cp_gi &x, const CP_GLOBAL uint64_t *tab, int g,
+                                             uint64_t J, uint64_t Jp, uint64_t p) {
+  const uint64_t t = cp_shoup(x.imag, J, Jp, p);
+  const CP_GLOBAL cp_u64x2 *k = (const CP_GLOBAL cp_u64x2 *)(tab + 4 * (size_t)g);
+  const cp_u64x2 kp = k[0], km = k[1];
+  cp_shoup2v(x.re + t, kp.x, kp.y, x.re + p - t, km.x, km.y, p, x.re, x.imag);
+}
+// Slot-wise product in the split basis.
+__device__ __forceinline__ cp_gi split_mul(const cp_gi &a, const cp_gi &b, const cp_prime &pr) {
+  return cp_gi{cp_mulmod(a.re, b.re, pr), cp_mulmod(a.imag, b.imag, pr)};
+}
+// (U, V) -> (c_g, c_{g+N}) = ((U + V) h, (U - V) hj), h = 1/2N, hj = 1/2JN.
+__device__ __forceinline__ void split_back(cp_gi &x, uint64_t h, uint64_t hp, uint64_t hj,
+                                           uint64_t hjp, uint64_t p) {
+  cp_shoup2(x.re + x.imag, h, hp, x.re + p - x.imag, hj, hjp, p, x.re, x.imag);
+}
+
+__device__ __forceinline__ void ld_root2(const CP_GLOBAL uint64_t *__restrict__ tg, unsigned node,
+                                         uint64_t &wu, uint64_t &wup, uint64_t &wv,
+                                         uint64_t &wvp) {
+  const CP_GLOBAL cp_u64x2 *k = (const CP_GLOBAL cp_u64x2 *)(tg + 4 * node);
+  const cp_u64x2 a = k[0], b = k[1];
+  wu = a.x;
+  wup = a.y;
+  wv = b.x;
+  wvp = b.y;
+}
+__device__ __forceinline__ void ld_root2(const uint64_t *__restrict__ tg, unsigned node,
+                                         uint64_t &wu, uint64_t &wup, uint64_t &wv,
+                                         uint64_t &wvp) {
+  const ulonglong2 *k = reinterpret_cast<const ulonglong2 *>(tg + 4 * node);
+  const ulonglong2 a = k[0], b = k[1];
+  wu = a.x;
+  wup = a.y;
+  wv = b.x;
+  wvp = b.y;
+}
+
+// One split-basis Gentleman-Sande butterfly: the group of A at global stage s
+// is pos >> (LOGN - s), its root at node 2^s + group.  At stage 0 the root is
+// the same for every butterfly (uniform, possibly in SGPRs).
+template <int LOGN, class TW>
+__device__ __forceinline__ void gsi_bfly(cp_gi &A, cp_gi &B, int s, int pos, TW __restrict__ tg,
+                                         uint64_t p) {
+  const unsigned node = (1u << s) + ((unsigned)pos >> (LOGN - s));
+  uint64_t wu, wup, wv, wvp;
+  ld_root2(tg, node, wu, wup, wv, wvp);
+  const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
+  A.re = cp_red2(A.re + B.re, p);
+  A.imag = cp_red2(A.imag + B.imag, p);
+  if (s == 0)
+    cp_shoup2(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+  else
+    cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+}
+
+// One split-basis inverse pass over local stages [LS0, LS0+R) in reverse
+// (the positions dit_pass pairs, in the same order).
+template <int LOGN, int LOGM, int LS0, int R, int SOFF, class Map, class TW>
+__device__ __forceinline__ void gsi_pass(cp_gi (&x)[EPT], int t, TW __restrict__ tg, uint64_t p,
+                                         const Map &map) {
+  constexpr int M = 1 << LOGM, T = M / EPT, U = 1 << R, ST = M >> (LS0 + R), NU = EPT / U;
+#pragma unroll
+  for (int ju = 0; ju < NU; ju++) {
+    const int u = t + ju * T;
+    const int base = (u / ST) * (U * ST) + (u % ST);
+#pragma unroll
+    for (int h = R - 1; h >= 0; h--) {
+      const int half = U >> (h + 1);
+      const int s = LS0 + h + SOFF;
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        if (q & half) continue;
+        gsi_bfly<LOGN>(x[ju * U + q], x[ju * U + q + half], s, map(base + q * ST), tg, p);
+      }
+      if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Full radix-8 split-basis inverse pass at a runtime local stage ls0.
+template <int LOGN, int LOGM, int SOFF, class Map, class TW>
+__device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restrict__ tg, uint64_t p,
+                                            const Map &map, int ls0) {
+  const int lst = LOGM - ls0 - LOG_EPT;
+  const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
+#pragma unroll
+  for (int h = LOG_EPT - 1; h >= 0; h--) {
+    const int half = EPT >> (h + 1);
+    const int s = ls0 + h + SOFF;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      if (q & half) continue;
+      const int pos = map(base + (q << lst));
+      const unsigned node = (1u << s) + ((unsigned)pos >> (LOGN - s));
+      uint64_t wu, wup, wv, wvp;
+      ld_root2(tg, node, wu, wup, wv, wvp);
+      cp_gi &A = x[q];
+      cp_gi &B = x[q + half];
+      const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
+      A.re = cp_red2(A.re + B.re, p);
+      A.imag = cp_red2(A.imag + B.imag, p);
+      cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+    }
+    if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Split-basis inverse passes over local stages [LSB, LSE) in reverse;
+// registers start in the last-pass layout and end in the first-pass layout
+// (the exchanges of dit_passes).
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW>
+__device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
+                                           TW __restrict__ tg, uint64_t p, const Map &map) {
+  using PP = PassPlan<LSB, LSE>;
+  constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
+  t = cp_opaque(t);
+  if constexpr (PP::NF > 0) gsi_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tg, p, map);
+  bool prev_local = false, prev_pad = false;
+#pragma nounroll
+  for (int i = PP::NF - 2; i >= 0; i--) {
+    const int ls0 = PP::LSF + LOG_EPT * i;
+    const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
+    const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
+    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
+    lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
+    if (local) wave_sync(); else __syncthreads();
+    lds_load_rt<LOGM>(lds, x, t, ls0, m);
+    gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0);
+    prev_local = local;
+    prev_pad = pad;
+  }
+  if constexpr (PP::R0 > 0) {
+    if constexpr (PP::NF > 0) {
+      __syncthreads();
+      lds_store_rt<LOGM>(lds, x, t, PP::LSF, MP);
+      __syncthreads();
+      lds_load<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
+    }
+    gsi_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tg, p, map);
+  }
+}
+
 // Minimum waves per SIMD asked of a one-transform-per-workgroup kernel: at
 // N = 4096 slots (512 threads, 72 KiB of LDS) two workgroups share a CU only
 // if each wave keeps to 128 VGPRs.

@@ -252,6 +252,59 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
       bi = ni;
     }
   }
+  // split-basis product tables (see dgt_dev.hpp): J = omega^(ng/4) squares to
+  // -1; xi+- = xi.re +- J xi.im; the inverse roots r^-1 of the split tree of
+  // X^ng -+ J come from its leaves xi+-^-1 omega^-bitrev(l) by squaring:
+  // r^-1 at (s, g) is the leaf-side value at (s + 1, 2g).
+  std::vector<uint64_t> spt((size_t)nres * ng * 4), spr((size_t)nres * ng * 4),
+      spk((size_t)nres * 8, 0);
+  {
+    int lg = 0;
+    while ((1 << lg) < ng) lg++;
+    std::vector<uint64_t> pw(ng), cur(ng), nxt(ng);
+    for (int r = 0; r < nres; r++) {
+      const uint64_t p = e->primes[r];
+      const uint64_t om = cp_powmod_host(cupoly_table_proot[r], (p - 1) / ng, p);
+      const uint64_t omi = invm(om, p);
+      const uint64_t J = cp_powmod_host(om, (uint64_t)ng / 4, p);
+      const uint64_t xr = cupoly_table_nthroot[r][si][0], xim = cupoly_table_nthroot[r][si][1];
+      const uint64_t xs[2] = {(xr + mulm(J, xim, p)) % p, (xr + p - mulm(J, xim, p)) % p};
+      if (mulm(J, J, p) != p - 1 || cp_powmod_host(xs[0], (uint64_t)ng, p) != J ||
+          cp_powmod_host(xs[1], (uint64_t)ng, p) != p - J) {
+        delete e;
+        return cp_fail(CUPOLY_ERR_UNSUPPORTED, "split basis: root tables inconsistent for this prime");
+      }
+      uint64_t f[2] = {1, 1};
+      for (int g = 0; g < ng; g++) {
+        for (int c = 0; c < 2; c++) {
+          shoup_pair(f[c], p, &spt[((size_t)r * ng + g) * 4 + 2 * c]);
+          f[c] = mulm(f[c], xs[c], p);
+        }
+      }
+      pw[0] = 1;
+      for (int x = 1; x < ng; x++) pw[x] = mulm(pw[x - 1], omi, p);
+      for (int c = 0; c < 2; c++) {
+        const uint64_t xinv = invm(xs[c], p);
+        for (int l = 0; l < ng; l++) {
+          int b = 0;
+          for (int i = 0; i < lg; i++) b |= ((l >> i) & 1) << (lg - 1 - i);
+          cur[l] = mulm(xinv, pw[b], p);
+        }
+        for (int s = lg - 1; s >= 0; s--) {
+          for (int g = 0; g < (1 << s); g++) {
+            const uint64_t ri = cur[2 * g];
+            shoup_pair(ri, p, &spr[((size_t)r * ng + (1 << s) + g) * 4 + 2 * c]);
+            nxt[g] = mulm(ri, ri, p);
+          }
+          std::swap(cur, nxt);
+        }
+      }
+      const uint64_t h = invm((uint64_t)(2 * ng) % p, p);
+      shoup_pair(J, p, &spk[(size_t)r * 8]);
+      shoup_pair(h, p, &spk[(size_t)r * 8 + 2]);
+      shoup_pair(mulm(h, invm(J, p), p), p, &spk[(size_t)r * 8 + 4]);
+    }
+  }
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
     CP_HIP(hipMalloc(dst, bytes));
     CP_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
@@ -263,6 +316,9 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
       (rc = up((void **)&e->d_tw_inv, twi.data(), twi.size() * 8)) ||
       (rc = up((void **)&e->d_twist_fwd, tsf.data(), tsf.size() * 8)) ||
       (rc = up((void **)&e->d_twist_inv, tsi.data(), tsi.size() * 8)) ||
+      (rc = up((void **)&e->d_split_tw, spt.data(), spt.size() * 8)) ||
+      (rc = up((void **)&e->d_split_root, spr.data(), spr.size() * 8)) ||
+      (rc = up((void **)&e->d_split_k, spk.data(), spk.size() * 8)) ||
       (rc = up((void **)&e->d_hps, &e->h_hps, sizeof(cp_hps))) ||
       (rc = up((void **)&e->d_crt, &e->h_crt, sizeof(cp_crt))) || (rc = cp_dgt_setup(e)) ||
       (rc = cp_homomul_setup(e)) || (rc = cp_encrypt_setup(e)) ||
@@ -278,8 +334,9 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
 extern "C" int cupoly_engine_destroy(cupoly_engine *e) {
   if (!e) return cp_fail(CUPOLY_ERR_ARG, "engine_destroy: null");
   (void)hipDeviceSynchronize();
-  void *ptrs[] = {e->d_primes, e->d_tw_fwd, e->d_tw_inv, e->d_twist_fwd, e->d_twist_inv,
-                  e->d_hps, e->d_crt};
+  void *ptrs[] = {e->d_primes,    e->d_tw_fwd,   e->d_tw_inv,       e->d_twist_fwd,
+                  e->d_twist_inv, e->d_split_tw, e->d_split_root, e->d_split_k,
+                  e->d_hps,       e->d_crt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete e;
