@@ -221,28 +221,34 @@ __device__ __forceinline__ P *uniform_ptr(P *q) {
   typedef __attribute__((address_space(1))) P global_p;
   return (P *)(global_p *)(((uint64_t)hi << 32) | lo);
 }
+// Loads and stores of the seven-transform kernel, in the split basis of
+// dgt_dev.hpp: the spectra never leave the workgroup except through its own
+// parks, so the basis does not show in Y.
 template <int LOGN>
 __device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restrict__ src, int t,
-                                         const uint64_t *__restrict__ tsf, uint64_t p) {
+                                         const CP_GLOBAL uint64_t *__restrict__ stw, uint64_t J,
+                                         uint64_t Jp, uint64_t p) {
   t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) x[i] = src[first_index<LOGN, 0, LOGN>(t, i)];
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    twist(x[i], tsf, first_index<LOGN, 0, LOGN>(t, i), p);
+  for (int i = 0; i < EPT; i += 2) {
+    split_twist2(x[i], x[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
+                 first_index<LOGN, 0, LOGN>(t, i + 1), J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 template <int LOGN>
 __device__ __forceinline__ void fti_store(cp_gi *__restrict__ dst, cp_gi (&x)[EPT], int t,
-                                          const uint64_t *__restrict__ tsi, uint64_t p,
+                                          const uint64_t *__restrict__ sk, uint64_t p,
                                           bool active) {
   t = cp_opaque(t);
+  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(t, i);
-      twist(x[i], tsi, g, p);
+      split_back(x[i], h, hp, hj, hjp, p);
       dst[g] = x[i];
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -253,8 +259,8 @@ template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_hom_fti(cp_gi *__restrict__ Y, const cp_gi *__restrict__ XB, int batch, int nres, int k,
               cp_cts ct1, cp_cts ct2, const cp_prime *__restrict__ primes,
-              const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ twi_all,
-              const uint64_t *__restrict__ tsf_all, const uint64_t *__restrict__ tsi_all) {
+              const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ root_all,
+              const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
@@ -269,8 +275,11 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const int pair = u - rid * batch;
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
-  const uint64_t *twf = twf_all + (size_t)rid * N, *twi = twi_all + (size_t)rid * N;
-  const uint64_t *tsf = tsf_all + (size_t)rid * N * 6, *tsi = tsi_all + (size_t)rid * N * 6;
+  const uint64_t *twf = twf_all + (size_t)rid * N;
+  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4);
+  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const uint64_t *sk = sk_all + (size_t)rid * 8;
+  const uint64_t J = sk[0], Jp = sk[1];
   // a Q residue comes straight from its ciphertext, a B residue from the
   // extension (XB = [pair][which][kl][N])
   const cp_gi *c0, *c1, *d0, *d1;
@@ -299,27 +308,27 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
-  fti_load<LOGN>(xa, c0, t, tsf, p);
+  fti_load<LOGN>(xa, c0, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c0, held
-  fti_load<LOGN>(xb, d0, t, tsf, p);
+  fti_load<LOGN>(xb, d0, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d0
   if (active) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) y1[i * T + cp_opaque(t)] = xb[i];
   }
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xb[i] = cp_gimul(xa[i], xb[i], pr);
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twi, p, id);
-  fti_store<LOGN>(y0, xb, t, tsi, p, active);
-  fti_load<LOGN>(xb, d1, t, tsf, p);
+  for (int i = 0; i < EPT; i++) xb[i] = split_mul(xa[i], xb[i], pr);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, root, p, id);
+  fti_store<LOGN>(y0, xb, t, sk, p, active);
+  fti_load<LOGN>(xb, d1, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d1, held
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
-    const cp_gi w = cp_gimul(xa[i], xb[i], pr);
+    const cp_gi w = split_mul(xa[i], xb[i], pr);
     if (active) y2[i * T + cp_opaque(t)] = w;
     __builtin_amdgcn_sched_barrier(0);
   }
-  fti_load<LOGN>(xa, c1, t, tsf, p);
+  fti_load<LOGN>(xa, c1, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c1
   const int tp = cp_opaque(t);
 #pragma unroll
@@ -327,16 +336,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     // one slot's park loads in flight at a time (all 16 hoisted would spill)
     const cp_gi pd0 = y1[i * T + tp], pc0d1 = y2[i * T + tp];
     const cp_gi cc = xa[i];
-    xb[i] = cp_gimul(cc, xb[i], pr);
-    const cp_gi w = cp_gimul(cc, pd0, pr);
+    xb[i] = split_mul(cc, xb[i], pr);
+    const cp_gi w = split_mul(cc, pd0, pr);
     xa[i].re = cp_add(w.re, pc0d1.re, p);
     xa[i].imag = cp_add(w.imag, pc0d1.imag, p);
     __builtin_amdgcn_sched_barrier(0);
   }
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twi, p, id);
-  fti_store<LOGN>(y1, xa, t, tsi, p, active);
-  dit_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twi, p, id);
-  fti_store<LOGN>(y2, xb, t, tsi, p, active);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
+  fti_store<LOGN>(y1, xa, t, sk, p, active);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, root, p, id);
+  fti_store<LOGN>(y2, xb, t, sk, p, active);
 }
 
 // ---- small batches: one thread per (slot, output residue) ------------------
@@ -568,7 +577,7 @@ static int launch_fti(cupoly_ctx *c, cp_gi *Y, const cp_gi *XB, int batch, cp_ct
   const int nunits = batch * e->nres;
   hipLaunchKernelGGL((k_hom_fti<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
                      c->stream, Y, XB, batch, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
-                     e->d_tw_inv, e->d_twist_fwd, e->d_twist_inv);
+                     e->d_split_root, e->d_split_tw, e->d_split_k);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }
