@@ -335,8 +335,8 @@ __global__ void __launch_bounds__(1024)
     k_polymul_16k(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
                   const cp_gi *__restrict__ b, int nunits, int nres,
                   const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                  const uint64_t *__restrict__ twi_all, const uint64_t *__restrict__ tsf_all,
-                  const uint64_t *__restrict__ tsi_all, cp_gi *__restrict__ park_all) {
+                  const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
+                  const uint64_t *__restrict__ sk_all, cp_gi *__restrict__ park_all) {
   constexpr int N = 16384;
   extern __shared__ cp_gi lds[];
   CP_GLOBAL cp_gi *park = (CP_GLOBAL cp_gi *)(park_all + (size_t)blockIdx.x * 3 * (N / 2));
@@ -347,9 +347,9 @@ __global__ void __launch_bounds__(1024)
     polymul_16k_unit((CP_GLOBAL cp_gi *)(out + off), (const CP_GLOBAL cp_gi *)(a + off),
                      (const CP_GLOBAL cp_gi *)(b + off), park, lds, pr,
                      (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N),
-                     (const CP_GLOBAL uint64_t *)(twi_all + (size_t)rid * N),
-                     (const CP_GLOBAL uint64_t *)(tsf_all + (size_t)rid * N * 6),
-                     (const CP_GLOBAL uint64_t *)(tsi_all + (size_t)rid * N * 6));
+                     (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4),
+                     (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4),
+                     sk_all + (size_t)rid * 8);
     __syncthreads();
   }
 }
@@ -542,8 +542,8 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     if (rc) return rc;
     const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     hipLaunchKernelGGL(k_polymul_16k, dim3(grid), dim3(1024), lds, c->stream, out, a, b, nunits,
-                       nres, e->d_primes, e->d_tw_fwd, e->d_tw_inv, e->d_twist_fwd,
-                       e->d_twist_inv, (cp_gi *)ws);
+                       nres, e->d_primes, e->d_tw_fwd, e->d_split_root, e->d_split_tw,
+                       e->d_split_k, (cp_gi *)ws);
     CP_LAUNCH_CHECK();
   } else {
     // out <- a, scratch <- b (caller inputs are not modified); transforms in place

@@ -681,18 +681,19 @@ __device__ __forceinline__ void park_ld(const CP_GLOBAL cp_gi *park, int area, i
   for (int i = 0; i < EPT; i++) x[i] = ld_gi(park + (area * EPT + i) * 1024 + t);
 }
 
-// twist + forward stage 0 of src; half 0 -> x0, half 1 -> x1
+// conversion and twist (split basis) + forward stage 0 of src; half 0 -> x0,
+// half 1 -> x1
 __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
                                            const CP_GLOBAL cp_gi *src, int t,
                                            const CP_GLOBAL uint64_t *twf,
-                                           const CP_GLOBAL uint64_t *tsf, uint64_t p) {
+                                           const CP_GLOBAL uint64_t *stw, uint64_t J, uint64_t Jp,
+                                           uint64_t p) {
   constexpr int H = 8192;
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
     cp_gi x = ld_gi(src + g), y = ld_gi(src + g + H);
-    twist(x, tsf, g, p);
-    twist(y, tsf, g + H, p);
+    split_twist2(x, y, stw, g, g + H, J, Jp, p);
     uint64_t w, wp;
     ld_tw(twf, (unsigned)g, w, wp);
     x0[i].re = cp_red2(x.re + y.re, p);
@@ -702,27 +703,27 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
   }
 }
 
-// inverse stage 0 (twiddle omega^-g) and untwist (xi^-n / N) of slot g and
-// g + 8192, written to out
+// split-tree inverse stage 0 (root r_0^-1 of node 1, the same for every
+// slot) of slots g and g + 8192 and the conversion back to coefficients,
+// written to out; sk = the residue's split constants
 __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&x)[EPT],
                                             const cp_gi (&y)[EPT], int t,
-                                            const CP_GLOBAL uint64_t *twi,
-                                            const CP_GLOBAL uint64_t *tsi, uint64_t p) {
+                                            const CP_GLOBAL uint64_t *root,
+                                            const uint64_t *sk, uint64_t p) {
   constexpr int H = 8192;
+  uint64_t wu, wup, wv, wvp;
+  ld_root2(root, 1u, wu, wup, wv, wvp);
+  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
-    uint64_t w, wp;
-    ld_tw(twi, (unsigned)g, w, wp);
-    uint64_t tr, ti;
-    cp_shoup2v(y[i].re, w, wp, y[i].imag, w, wp, p, tr, ti);
     cp_gi s, dd;
-    s.re = cp_red2(x[i].re + tr, p);
-    s.imag = cp_red2(x[i].imag + ti, p);
-    dd.re = cp_red2(x[i].re + p - tr, p);
-    dd.imag = cp_red2(x[i].imag + p - ti, p);
-    twist(s, tsi, g, p);
-    twist(dd, tsi, g + H, p);
+    s.re = cp_red2(x[i].re + y[i].re, p);
+    s.imag = cp_red2(x[i].imag + y[i].imag, p);
+    cp_shoup2(x[i].re + p - y[i].re, wu, wup, x[i].imag + p - y[i].imag, wv, wvp, p, dd.re,
+              dd.imag);
+    split_back(s, h, hp, hj, hjp, p);
+    split_back(dd, h, hp, hj, hjp, p);
     st_gi(out + g, s);
     st_gi(out + g + H, dd);
     __builtin_amdgcn_sched_barrier(0);
@@ -735,15 +736,16 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
                                                  const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
                                                  cp_gi *lds, const cp_prime &pr,
                                                  const CP_GLOBAL uint64_t *twf,
-                                                 const CP_GLOBAL uint64_t *twi,
-                                                 const CP_GLOBAL uint64_t *tsf,
-                                                 const CP_GLOBAL uint64_t *tsi) {
+                                                 const CP_GLOBAL uint64_t *root,
+                                                 const CP_GLOBAL uint64_t *stw,
+                                                 const uint64_t *sk) {
   constexpr int H = 8192;
   const int t = (int)threadIdx.x;
   const uint64_t p = pr.p;
+  const uint64_t J = sk[0], Jp = sk[1];
   cp_gi x[EPT], y[EPT];
   // a: spectrum halves to P0, P1
-  stage0_16k(x, y, a, t, twf, tsf, p);
+  stage0_16k(x, y, a, t, twf, stw, J, Jp, p);
   park_st(park, 1, t, y);
 #pragma nounroll
   for (int h = 0; h < 2; h++) {
@@ -755,7 +757,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   // half 0's result waits in P0
   int tb = t;
   asm volatile("" : "+v"(tb));
-  stage0_16k(x, y, b, tb, twf, tsf, p);
+  stage0_16k(x, y, b, tb, twf, stw, J, Jp, p);
   park_st(park, 2, t, y);
 #pragma nounroll
   for (int h = 0; h < 2; h++) {
@@ -763,8 +765,8 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
     dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{h * H});
     park_ld(park, h, t, y);
 #pragma unroll
-    for (int i = 0; i < EPT; i++) x[i] = cp_gimul(x[i], y[i], pr);
-    dit_passes<14, 13, 0, 13, 1>(x, lds, t, twi, p, MapBlock{h * H});
+    for (int i = 0; i < EPT; i++) x[i] = split_mul(x[i], y[i], pr);
+    gsi_passes<14, 13, 0, 13, 1>(x, lds, t, root, p, MapBlock{h * H});
     if (h == 0) park_st(park, 0, t, x);
   }
 #pragma unroll
@@ -772,5 +774,5 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   park_ld(park, 0, t, x);
   int to = t;
   asm volatile("" : "+v"(to));
-  istage0_16k(c, x, y, to, twi, tsi, p);
+  istage0_16k(c, x, y, to, root, sk, p);
 }

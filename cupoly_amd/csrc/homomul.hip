@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(1024)
     k_dgt_fwd_tm16(cp_gi *__restrict__ X, int nunits, int nres, int k, cp_cts ct1, cp_cts ct2,
                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                   const uint64_t *__restrict__ tsf_all) {
+                   const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ sk_all) {
   constexpr int N = 16384, H = 8192;
   extern __shared__ cp_gi lds[];
   const int t = (int)threadIdx.x;
@@ -507,7 +507,8 @@ __global__ void __launch_bounds__(1024)
   const int unit = (u - rid * per_r) * nres + rid;
   const uint64_t p = primes[rid].p;
   const CP_GLOBAL uint64_t *twf = (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N);
-  const CP_GLOBAL uint64_t *tsf = (const CP_GLOBAL uint64_t *)(tsf_all + (size_t)rid * N * 6);
+  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const uint64_t J = sk_all[(size_t)rid * 8], Jp = sk_all[(size_t)rid * 8 + 1];
   CP_GLOBAL cp_gi *poly = (CP_GLOBAL cp_gi *)(X + (size_t)unit * N);
   const CP_GLOBAL cp_gi *src = poly;
   if (rid < k) {  // a Q residue comes straight from its ciphertext
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(1024)
                                      (size_t)((which & 1) * k + rid) * N);
   }
   cp_gi x[EPT], y[EPT];
-  stage0_16k(x, y, src, t, twf, tsf, p);
+  stage0_16k(x, y, src, t, twf, stw, J, Jp, p);
   park_st(poly, 1, t, y);
   dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0});
   park_st(poly, 0, t, x);
@@ -527,8 +528,8 @@ __global__ void __launch_bounds__(1024)
 
 __global__ void __launch_bounds__(1024)
     k_tensor_idgt16(cp_gi *__restrict__ Y, const cp_gi *__restrict__ X, int nunits, int nres,
-                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twi_all,
-                    const uint64_t *__restrict__ tsi_all) {
+                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ root_all,
+                    const uint64_t *__restrict__ sk_all) {
   constexpr int N = 16384, H = 8192;
   extern __shared__ cp_gi lds[];
   const int t = (int)threadIdx.x;
@@ -538,8 +539,8 @@ __global__ void __launch_bounds__(1024)
   const int pair = pm / 3, m = pm % 3;
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
-  const CP_GLOBAL uint64_t *twi = (const CP_GLOBAL uint64_t *)(twi_all + (size_t)rid * N);
-  const CP_GLOBAL uint64_t *tsi = (const CP_GLOBAL uint64_t *)(tsi_all + (size_t)rid * N * 6);
+  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4);
+  const uint64_t *sk = sk_all + (size_t)rid * 8;
   const size_t per_pair = (size_t)nres * N;
   const CP_GLOBAL cp_gi *x = (const CP_GLOBAL cp_gi *)(X + (size_t)pair * 4 * per_pair + (size_t)rid * N);
   const CP_GLOBAL cp_gi *xa = x + (m == 2 ? per_pair : 0), *xb = x + (m == 0 ? 2 : 3) * per_pair;
@@ -550,14 +551,14 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       const int g = h * H + i * 1024 + t;  // thread-major
-      v[i] = cp_gimul(ld_gi(xa + g), ld_gi(xb + g), pr);
+      v[i] = split_mul(ld_gi(xa + g), ld_gi(xb + g), pr);
       if (m == 1) {  // + c1 d0
-        const cp_gi z = cp_gimul(ld_gi(x + per_pair + g), ld_gi(x + 2 * per_pair + g), pr);
+        const cp_gi z = split_mul(ld_gi(x + per_pair + g), ld_gi(x + 2 * per_pair + g), pr);
         v[i].re = cp_add(v[i].re, z.re, p);
         v[i].imag = cp_add(v[i].imag, z.imag, p);
       }
     }
-    dit_passes<14, 13, 0, 13, 1>(v, lds, t, twi, p, MapBlock{h * H});
+    gsi_passes<14, 13, 0, 13, 1>(v, lds, t, root, p, MapBlock{h * H});
     if (h == 0) park_st(y, 0, t, v);  // half 0 waits in its own final place
   }
 #pragma unroll
@@ -565,7 +566,7 @@ __global__ void __launch_bounds__(1024)
   park_ld(y, 0, t, v);
   int to = t;
   asm volatile("" : "+v"(to));
-  istage0_16k(y, v, w, to, twi, tsi, p);
+  istage0_16k(y, v, w, to, root, sk, p);
 }
 
 template <int LOGN>
@@ -825,10 +826,10 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     const size_t lds = lds_row<13>() * sizeof(cp_gi);
     hipLaunchKernelGGL(k_dgt_fwd_tm16, dim3(batch * 4 * nres), dim3(1024), lds, c->stream, X,
                        batch * 4 * nres, nres, k, ct1, ct2, e->d_primes, e->d_tw_fwd,
-                       e->d_twist_fwd);
+                       e->d_split_tw, e->d_split_k);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tensor_idgt16, dim3(batch * 3 * nres), dim3(1024), lds, c->stream, Y, X,
-                       batch * 3 * nres, nres, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+                       batch * 3 * nres, nres, e->d_primes, e->d_split_root, e->d_split_k);
     CP_LAUNCH_CHECK();
   } else {
     if ((rc = cp_dgt_launch(c, X, 0, nres, batch * 4, CUPOLY_FORWARD))) return rc;
