@@ -142,8 +142,7 @@ __global__ void __launch_bounds__(256)
     const cp_gi yv = yb[(size_t)i * ng + c];
     uint64_t rr, ri;
     if (!slow) {
-      rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
-      ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
+      scale_exact2(hr, hi, xr, xi, K, H->wint[i], yv.re, yv.imag, H->lam[i], b, rr, ri);
     } else {  // the reference's chain, operation for operation
       rr = scale_chain_word(xr, fr, K, H->wint[i], round_r, yv.re, H->lam[i], b);
       ri = scale_chain_word(xi, fi, K, H->wint[i], round_i, yv.imag, H->lam[i], b);
@@ -156,8 +155,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < K; j++) {
       if constexpr (kAcc) {
-        cp_acc_mac(ar_acc[j], ar, H->Bbiqi[j][i][0]);
-        cp_acc_mac(ai_acc[j], ai, H->Bbiqi[j][i][0]);
+        cp_acc_mac2(ar_acc[j], ai_acc[j], ar, ai, H->Bbiqi[j][i][0]);
       } else {
         const uint64_t q = H->q[j].p;
         sr[j] = addmod_ref(sr[j], shoup2(ar, H->Bbiqi[j][i], q), q);
@@ -173,8 +171,7 @@ __global__ void __launch_bounds__(256)
     cp_gi r;
     if constexpr (kAcc) {
       const uint64_t bq = H->Bqi[j][0], nbq = bq ? q - bq : 0;  // -B mod q_j
-      cp_acc_mac(ar_acc[j], v_re, nbq);
-      cp_acc_mac(ai_acc[j], v_im, nbq);
+      cp_acc_mac2(ar_acc[j], ai_acc[j], v_re, v_im, nbq);
       r.re = cp_acc_reduce(ar_acc[j], q);
       r.imag = cp_acc_reduce(ai_acc[j], q);
     } else {
@@ -380,13 +377,8 @@ __global__ void __launch_bounds__(256)
     }
   }
 #pragma unroll
-  for (int j = 0; j < CP_MAX_BASE; j++) {
-    if (j < k) {
-      const double r = H->rcp_q[j];
-      vr = __ocml_fma_rtn_f64((double)ar[j], r, vr);
-      vi = __ocml_fma_rtn_f64((double)ai[j], r, vi);
-    }
-  }
+  for (int j = 0; j < CP_MAX_BASE; j++)
+    if (j < k) cp_fma_rd2((double)ar[j], (double)ai[j], H->rcp_q[j], vr, vi);
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   // exact sum, reduced once (as hps_q_to_b_slot)
   const uint64_t b = H->b[i].p, qb = H->Qbi[i][0];
@@ -397,12 +389,10 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int j = 0; j < CP_MAX_BASE; j++) {
     if (j < k) {
-      cp_acc_mac(sr, ar[j], H->Qqibi[i][j][0]);
-      cp_acc_mac(si, ai[j], H->Qqibi[i][j][0]);
+      cp_acc_mac2(sr, si, ar[j], ai[j], H->Qqibi[i][j][0]);
     }
   }
-  cp_acc_mac(sr, v_re, nqb);
-  cp_acc_mac(si, v_im, nqb);
+  cp_acc_mac2(sr, si, v_re, v_im, nqb);
   out[(size_t)(boff + i) * ng + c] = cp_gi{cp_acc_reduce(sr, b), cp_acc_reduce(si, b)};
 }
 
@@ -445,8 +435,7 @@ __global__ void __launch_bounds__(256)
   const cp_gi yv = y[(size_t)(K + i) * ng + c];
   uint64_t rr, ri;
   if (!slow) {  // same word as the reference's chain (scale_exact)
-    rr = scale_exact(hr, xr, K, H->wint[i], yv.re, H->lam[i], b);
-    ri = scale_exact(hi, xi, K, H->wint[i], yv.imag, H->lam[i], b);
+    scale_exact2(hr, hi, xr, xi, K, H->wint[i], yv.re, yv.imag, H->lam[i], b, rr, ri);
   } else {
     rr = scale_chain_word(xr, fr, K, H->wint[i], round_r, yv.re, H->lam[i], b);
     ri = scale_chain_word(xi, fi, K, H->wint[i], round_i, yv.imag, H->lam[i], b);
@@ -473,13 +462,11 @@ __global__ void __launch_bounds__(256)
     const double bd = (double)H->b[i].p;
     vr = vr + (double)a.re / bd;
     vi = vi + (double)a.imag / bd;
-    cp_acc_mac(sr, a.re, H->Bbiqi[j][i][0]);
-    cp_acc_mac(si, a.imag, H->Bbiqi[j][i][0]);
+    cp_acc_mac2(sr, si, a.re, a.imag, H->Bbiqi[j][i][0]);
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   const uint64_t bq = H->Bqi[j][0], nbq = bq ? q - bq : 0;  // -B mod q_j
-  cp_acc_mac(sr, v_re, nbq);
-  cp_acc_mac(si, v_im, nbq);
+  cp_acc_mac2(sr, si, v_re, v_im, nbq);
   cp_gi r;
   r.re = cp_acc_reduce(sr, q);
   r.imag = cp_acc_reduce(si, q);

@@ -16,7 +16,7 @@
 //   Q->B v:   value = RD(RN(aux_j) * RD(1/RN(q_j)) + value), ascending j
 //             (__fma_rd / __drcp_rd at cudaengine.cu:527); the reciprocal is a
 //             per-prime constant and the FMA runs in round-toward-minus-
-//             infinity mode (__ocml_fma_rtn_f64 sets the MODE register);
+//             infinity mode (cp_fma_rd2 sets the MODE register);
 //   B->Q v:   value = RN(value + RN(RN(aux_i) / RN(b_i))), ascending i (:700);
 //   scaling:  acc = RN(RN(lo) * 2^-64 + acc) (:413-414), then llrint;
 //   snap:     |value - llrint(value)| <= 1e-9 ? llrint(value) : floor(value)
@@ -58,12 +58,8 @@ __global__ void __launch_bounds__(64 * CP_MAX_BASE)
     }
     // v = RD(aux_j * RD(1/q_j) + v), ascending j (cudaengine.cu:526-527)
 #pragma unroll
-    for (int j = 0; j < CP_MAX_BASE; j++) {
-      if (j < k) {
-        vr = __ocml_fma_rtn_f64((double)ar[j], H->rcp_q[j], vr);
-        vi = __ocml_fma_rtn_f64((double)ai[j], H->rcp_q[j], vi);
-      }
-    }
+    for (int j = 0; j < CP_MAX_BASE; j++)
+      if (j < k) cp_fma_rd2((double)ar[j], (double)ai[j], H->rcp_q[j], vr, vi);
   }
   __syncthreads();
   if (!on) return;
@@ -77,12 +73,10 @@ __global__ void __launch_bounds__(64 * CP_MAX_BASE)
 #pragma unroll
   for (int j = 0; j < CP_MAX_BASE; j++) {
     if (j < k) {
-      cp_acc_mac(sr, ar[j], H->Qqibi[i][j][0]);
-      cp_acc_mac(si, ai[j], H->Qqibi[i][j][0]);
+      cp_acc_mac2(sr, si, ar[j], ai[j], H->Qqibi[i][j][0]);
     }
   }
-  cp_acc_mac(sr, v_re, nqb);
-  cp_acc_mac(si, v_im, nqb);
+  cp_acc_mac2(sr, si, v_re, v_im, nqb);
   a[(size_t)i * ng + c] = cp_gi{cp_acc_reduce(sr, b), cp_acc_reduce(si, b)};
 }
 
@@ -126,8 +120,7 @@ __global__ void __launch_bounds__(256)
   const cp_gi y = ab[(size_t)i * ng + c];
   uint64_t rr, ri;
   if (!slow) {
-    rr = scale_exact(hr, xr, K, H->wint[i], y.re, H->lam[i], b);
-    ri = scale_exact(hi, xi, K, H->wint[i], y.imag, H->lam[i], b);
+    scale_exact2(hr, hi, xr, xi, K, H->wint[i], y.re, y.imag, H->lam[i], b, rr, ri);
   } else {
     rr = scale_chain_word(xr, fr, K, H->wint[i], round_r, y.re, H->lam[i], b);
     ri = scale_chain_word(xi, fi, K, H->wint[i], round_i, y.imag, H->lam[i], b);
@@ -156,13 +149,11 @@ __global__ void __launch_bounds__(256)
     const double bd = (double)b;
     vr = vr + (double)ar / bd;
     vi = vi + (double)ai / bd;
-    cp_acc_mac(sr, ar, H->Bbiqi[j][i][0]);
-    cp_acc_mac(si, ai, H->Bbiqi[j][i][0]);
+    cp_acc_mac2(sr, si, ar, ai, H->Bbiqi[j][i][0]);
   }
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   const uint64_t bq = H->Bqi[j][0], nbq = bq ? q - bq : 0;  // -B mod q_j
-  cp_acc_mac(sr, v_re, nbq);
-  cp_acc_mac(si, v_im, nbq);
+  cp_acc_mac2(sr, si, v_re, v_im, nbq);
   out[(size_t)j * ng + c] = cp_gi{cp_acc_reduce(sr, q), cp_acc_reduce(si, q)};
 }
 

@@ -9,7 +9,25 @@
 
 #include "internal.hpp"
 
-extern "C" __device__ double __ocml_fma_rtn_f64(double, double, double);
+// Two FP64 fused multiply-adds rounded toward -infinity (CUDA's __fma_rd, the
+// reference's rounding of the HPS fractional sums): vr = RD(xr r + vr),
+// vi = RD(xi r + vi).  The MODE register's FP64 rounding field (bits 3:2) is
+// set to round-down for the two instructions and back to round-to-nearest in
+// the same asm block, so no other floating-point instruction runs under the
+// directed mode (the compiler's own mode tracking does not look inside asm).
+// A VALU instruction may follow a MODE write at once; a MODE write after a
+// VALU instruction needs one wait state, hence the s_nop.
+__device__ __forceinline__ void cp_fma_rd2(double xr, double xi, double r, double &vr,
+                                           double &vi) {
+  asm volatile(
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 2\n\t"
+      "v_fma_f64 %0, %2, %4, %0\n\t"
+      "v_fma_f64 %1, %3, %4, %1\n\t"
+      "s_nop 0\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0"
+      : "+&v"(vr), "+&v"(vi)
+      : "v"(xr), "v"(xi), "v"(r));
+}
 
 // The reference's addmod / submod macros (include/cuPoly/cuda/dgt.h:73-74):
 // one conditional correction on 64-bit wrap-around words, kept verbatim for
@@ -61,13 +79,8 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
   }
   // v = RD(aux_j * RD(1/q_j) + v) in ascending j (cudaengine.cu:526-527)
 #pragma unroll
-  for (int j = 0; j < CP_MAX_BASE; j++) {
-    if (j < k) {
-      const double r = H->rcp_q[j];
-      vr = __ocml_fma_rtn_f64((double)ar[j], r, vr);
-      vi = __ocml_fma_rtn_f64((double)ai[j], r, vi);
-    }
-  }
+  for (int j = 0; j < CP_MAX_BASE; j++)
+    if (j < k) cp_fma_rd2((double)ar[j], (double)ai[j], H->rcp_q[j], vr, vi);
   const uint64_t v_re = snap(vr), v_im = snap(vi);
   for (int i = 0; i < kl; i++) {
     // (sum_j a_j (Q/q_j mod b_i) - v (Q mod b_i)) mod b_i: every term is
@@ -79,14 +92,9 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
     cp_acc_init(sr);
     cp_acc_init(si);
 #pragma unroll
-    for (int j = 0; j < CP_MAX_BASE; j++) {
-      if (j < k) {
-        cp_acc_mac(sr, ar[j], H->Qqibi[i][j][0]);
-        cp_acc_mac(si, ai[j], H->Qqibi[i][j][0]);
-      }
-    }
-    cp_acc_mac(sr, v_re, nqb);
-    cp_acc_mac(si, v_im, nqb);
+    for (int j = 0; j < CP_MAX_BASE; j++)
+      if (j < k) cp_acc_mac2(sr, si, ar[j], ai[j], H->Qqibi[i][j][0]);
+    cp_acc_mac2(sr, si, v_re, v_im, nqb);
     cp_gi o;
     o.re = cp_acc_reduce(sr, b);
     o.imag = cp_acc_reduce(si, b);
@@ -94,8 +102,8 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
   }
 }
 
-// Complex scaling t/Q, residue i of one slot (cudaengine.cu:401-433): the
-// reference forms
+// Complex scaling t/Q, residue i of both coefficients of one slot
+// (cudaengine.cu:401-433): the reference forms
 //     y_i = sum_j (x_j w_ij mod b_i  (+)  h_j)  (+)  round  (+)  lambda_i xB_i mod b_i
 // with its single-correction addmod (+), where h_j = floor(x_j omega_frac_j
 // / 2^128) < q_j and round = llrint(sum_j RN(l_j 2^-64)) <= k.  When every
@@ -105,14 +113,17 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
 // An h_j >= b_i needs x_j within 2^30 of q_j (the primes lie within 2^30 of
 // 2^63): about one coefficient in 2^33.  Such a slot takes the reference's
 // chain as written (`slow`), so the result is the same word in every case.
-__device__ __forceinline__ uint64_t scale_exact(const cp_acc &hsum, const uint64_t *x, int kq,
-                                                const uint64_t (*wint)[2], uint64_t xb,
-                                                const uint64_t (&lam)[2], uint64_t b) {
-  cp_acc a = hsum;
+__device__ __forceinline__ void scale_exact2(const cp_acc &hsum_r, const cp_acc &hsum_i,
+                                             const uint64_t *xr, const uint64_t *xi, int kq,
+                                             const uint64_t (*wint)[2], uint64_t xbr,
+                                             uint64_t xbi, const uint64_t (&lam)[2], uint64_t b,
+                                             uint64_t &rr, uint64_t &ri) {
+  cp_acc ar = hsum_r, ai = hsum_i;
 #pragma unroll
-  for (int j = 0; j < kq; j++) cp_acc_mac(a, x[j], wint[j][0]);
-  cp_acc_mac(a, xb, lam[0]);
-  return cp_acc_reduce(a, b);
+  for (int j = 0; j < kq; j++) cp_acc_mac2(ar, ai, xr[j], xi[j], wint[j][0]);
+  cp_acc_mac2(ar, ai, xbr, xbi, lam[0]);
+  rr = cp_acc_reduce(ar, b);
+  ri = cp_acc_reduce(ai, b);
 }
 
 // hsum = sum_j h_j + round as a 160-bit accumulator value

@@ -89,8 +89,8 @@ __device__ __forceinline__ void cp_mulhi2v(uint64_t x, uint64_t bx, uint64_t y, 
   hy = ((uint64_t)uy << 32) | (uint32_t)ty;
 }
 #else
-static inline void cp_mulhi2v(uint64_t x, uint64_t bx, uint64_t y, uint64_t by, uint64_t &hx,
-                              uint64_t &hy) {
+CP_HD void cp_mulhi2v(uint64_t x, uint64_t bx, uint64_t y, uint64_t by, uint64_t &hx,
+                      uint64_t &hy) {
   cp_mulhi2(x, bx, y, by, hx, hy);
 }
 #endif
@@ -216,6 +216,74 @@ CP_HD void cp_acc_mac(cp_acc &s, uint64_t a, uint64_t b) {
   s.hi = cp_addc64(s.hi, (uint64_t)(x >> 64), c);
   s.top += c;
 }
+
+// Two accumulations by one multiplier, sx += x b and sy += y b, b < 2^63.
+// On the GPU the product's middle column and the accumulator add are chained
+// through the carry-outs of v_mad_u64_u32 and add-with-carry instead of the
+// 64-bit compares the compiler makes of `s < a`:
+//   P  = x0 b0 + lo            (carry c0 at 2^64)
+//   M  = x0 b1 + P.hi          (< 2^64)
+//   M2 = x1 b0 + M             (carry c1 at 2^96)
+//   T  = x1 b1 + M2.hi + c1 2^32  (< 2^63 + 2^33, as b1 < 2^31)
+//   lo = (P.lo, M2.lo),  hi += T + c0,  top += carry.
+// The x and y chains alternate, and each carry is read at least two
+// instructions (or an s_nop) after its write: gfx950 needs two wait states
+// between a VALU write of a lane mask and a VALU read of it as a carry.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void cp_acc_mac2(cp_acc &sx, cp_acc &sy, uint64_t x, uint64_t y,
+                                            uint64_t b) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  uint64_t px, py, cx0, cy0;
+  asm("v_mad_u64_u32 %0, %2, %4, %6, %7\n\t"
+      "v_mad_u64_u32 %1, %3, %5, %6, %8"
+      : "=&v"(px), "=&v"(py), "=&s"(cx0), "=&s"(cy0)
+      : "v"(x0), "v"(y0), "v"(b0), "v"(sx.lo), "v"(sy.lo));
+  const uint64_t mx = (uint64_t)x0 * b1 + (px >> 32);
+  const uint64_t my = (uint64_t)y0 * b1 + (py >> 32);
+  uint64_t m2x, m2y, cx1, cy1;
+  asm("v_mad_u64_u32 %0, %2, %4, %6, %7\n\t"
+      "v_mad_u64_u32 %1, %3, %5, %6, %8"
+      : "=&v"(m2x), "=&v"(m2y), "=&s"(cx1), "=&s"(cy1)
+      : "v"(x1), "v"(y1), "v"(b0), "v"(mx), "v"(my));
+  const uint64_t tx = (uint64_t)x1 * b1 + (m2x >> 32);
+  const uint64_t ty = (uint64_t)y1 * b1 + (m2y >> 32);
+  uint32_t txl = (uint32_t)tx, txh = (uint32_t)(tx >> 32);
+  uint32_t tyl = (uint32_t)ty, tyh = (uint32_t)(ty >> 32);
+  uint32_t hxl = (uint32_t)sx.hi, hxh = (uint32_t)(sx.hi >> 32);
+  uint32_t hyl = (uint32_t)sy.hi, hyh = (uint32_t)(sy.hi >> 32);
+  uint32_t topx = sx.top, topy = sy.top;
+  uint64_t j0, j1, e1x, e1y, e2x, e2y, j2, j3;
+  asm("v_addc_co_u32_e64 %[txh], %[j0], %[txh], 0, %[cx1]\n\t"
+      "v_addc_co_u32_e64 %[tyh], %[j1], %[tyh], 0, %[cy1]\n\t"
+      "v_addc_co_u32_e64 %[hxl], %[e1x], %[hxl], %[txl], %[cx0]\n\t"
+      "v_addc_co_u32_e64 %[hyl], %[e1y], %[hyl], %[tyl], %[cy0]\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %[hxh], %[e2x], %[hxh], %[txh], %[e1x]\n\t"
+      "v_addc_co_u32_e64 %[hyh], %[e2y], %[hyh], %[tyh], %[e1y]\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %[topx], %[j2], %[topx], 0, %[e2x]\n\t"
+      "v_addc_co_u32_e64 %[topy], %[j3], %[topy], 0, %[e2y]"
+      : [txh] "+&v"(txh), [tyh] "+&v"(tyh), [hxl] "+&v"(hxl), [hyl] "+&v"(hyl),
+        [hxh] "+&v"(hxh), [hyh] "+&v"(hyh), [topx] "+&v"(topx), [topy] "+&v"(topy),
+        [j0] "=&s"(j0), [j1] "=&s"(j1), [e1x] "=&s"(e1x), [e1y] "=&s"(e1y), [e2x] "=&s"(e2x),
+        [e2y] "=&s"(e2y), [j2] "=&s"(j2), [j3] "=&s"(j3)
+      : [txl] "v"(txl), [tyl] "v"(tyl), [cx0] "s"(cx0), [cy0] "s"(cy0), [cx1] "s"(cx1),
+        [cy1] "s"(cy1));
+  sx.lo = ((uint64_t)(uint32_t)m2x << 32) | (uint32_t)px;
+  sy.lo = ((uint64_t)(uint32_t)m2y << 32) | (uint32_t)py;
+  sx.hi = ((uint64_t)hxh << 32) | hxl;
+  sy.hi = ((uint64_t)hyh << 32) | hyl;
+  sx.top = topx;
+  sy.top = topy;
+}
+#else
+CP_HD void cp_acc_mac2(cp_acc &sx, cp_acc &sy, uint64_t x, uint64_t y, uint64_t b) {
+  cp_acc_mac(sx, x, b);
+  cp_acc_mac(sy, y, b);
+}
+#endif
 
 CP_HD uint64_t cp_acc_reduce(const cp_acc &s, uint64_t p) {
   const uint64_t delta = (1ULL << 63) - p;
