@@ -34,7 +34,8 @@
 
 struct cupoly_relin_key {
   cupoly_engine *e = nullptr;
-  cp_gi *d = nullptr;  // [k][2][k][ng], DGT domain: [i][0] = b_i, [i][1] = a_i
+  cp_gi *d = nullptr;   // [k][2][k][ng], DGT domain: [i][0] = b_i, [i][1] = a_i
+  cp_gi *ds = nullptr;  // the same in the split basis (dgt_dev.hpp), for k_relin_fti
 };
 
 // Where polynomial-set i lives: base + i * stride, or table[i].
@@ -69,6 +70,24 @@ __global__ void __launch_bounds__(256)
     b.imag = cp_add(b.imag, cp_shoup(v.imag, H->Qqi[i][0], H->Qqi[i][1], p), p);
   }
   key[((size_t)(2 * i) * k) * ng + row] = b;
+}
+
+// The key in the split basis of dgt_dev.hpp: a DGT-domain slot X of residue j
+// becomes (X.re + J X.im, X.re - J X.im) mod q_j.  a + ib -> a +- Jb is a ring
+// homomorphism Z_p[i] -> Z_p, so this is the split-basis spectrum of the same
+// polynomial, position for position.  blockIdx.y = row of the key (i, m, j),
+// j = blockIdx.y % k.
+__global__ void __launch_bounds__(256)
+    k_relin_key_split(cp_gi *__restrict__ ds, const cp_gi *__restrict__ d, int ng,
+                      const cp_hps *__restrict__ H, const uint64_t *__restrict__ sk_all) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ng) return;
+  const int j = blockIdx.y % H->k;
+  const uint64_t p = H->q[j].p, *sk = sk_all + (size_t)j * 8;
+  const size_t at = (size_t)blockIdx.y * ng + c;
+  const cp_gi x = d[at];
+  const uint64_t t = cp_shoup(x.imag, sk[0], sk[1], p);
+  ds[at] = cp_gi{cp_add(x.re, t, p), cp_sub(x.re, t, p)};
 }
 
 // Digit i of coefficient-domain word x (residue i of y2), in residue j.
@@ -131,12 +150,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- fused form, n = 2^11 .. 2^14 ---------------------------------------------
+// In the split basis of dgt_dev.hpp (digits converted and twisted, products
+// with the split-basis key, split-tree inverses): nothing of the transform
+// domain leaves the workgroup but the parked a-sum, which only it reads.
 template <int LOGN, int ROWS>
 __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LOGN>())
     k_relin_fti(cp_rcts out, const cp_gi *__restrict__ T, const cp_gi *__restrict__ key,
                 cp_gi *__restrict__ park_all, int batch, const cp_hps *__restrict__ H,
-                const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ twi_all,
-                const uint64_t *__restrict__ tsf_all, const uint64_t *__restrict__ tsi_all) {
+                const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ root_all,
+                const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, TT = N / EPT;
   using LP = LastPass<LOGN, 0, LOGN>;
   extern __shared__ cp_gi lds_all[];
@@ -153,8 +175,11 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const int pair = u - j * batch;
   const cp_prime pr = H->q[j];
   const uint64_t p = pr.p;
-  const uint64_t *twf = twf_all + (size_t)j * N, *twi = twi_all + (size_t)j * N;
-  const uint64_t *tsf = tsf_all + (size_t)j * N * 6, *tsi = tsi_all + (size_t)j * N * 6;
+  const uint64_t *twf = twf_all + (size_t)j * N;
+  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)j * N * 4);
+  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)j * N * 4);
+  const uint64_t *sk = sk_all + (size_t)j * 8;
+  const uint64_t J = sk[0], Jp = sk[1];
   const cp_gi *y = T + (size_t)pair * 3 * k * N;  // y0, y1, y2 of this pair, [3][k][N]
   cp_gi *park = park_all + (size_t)u * N;          // the a-sum, thread-major
   MapIdentity id;
@@ -164,11 +189,12 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     const int tl = cp_opaque(t);
     const cp_gi *y2i = y + ((size_t)2 * k + i) * N;
 #pragma unroll
-    for (int r = 0; r < EPT; r++) {
-      const int g = first_index<LOGN, 0, LOGN>(tl, r);
-      const cp_gi v = y2i[g];
-      x[r] = cp_gi{relin_digit(v.re, H, i, p), relin_digit(v.imag, H, i, p)};
-      twist(x[r], tsf, g, p);
+    for (int r = 0; r < EPT; r += 2) {
+      const int g0 = first_index<LOGN, 0, LOGN>(tl, r), g1 = first_index<LOGN, 0, LOGN>(tl, r + 1);
+      const cp_gi v0 = y2i[g0], v1 = y2i[g1];
+      x[r] = cp_gi{relin_digit(v0.re, H, i, p), relin_digit(v0.imag, H, i, p)};
+      x[r + 1] = cp_gi{relin_digit(v1.re, H, i, p), relin_digit(v1.imag, H, i, p)};
+      split_twist2(x[r], x[r + 1], stw, g0, g1, J, Jp, p);
       __builtin_amdgcn_sched_barrier(0);
     }
     dif_passes<LOGN, LOGN, 0, LOGN, 0>(x, lds, t, twf, p, id);
@@ -177,7 +203,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
       const int pos = pass_index<LOGN, LP::LS0, LP::RL>(tk, r);
-      const cp_gi m0 = cp_gimul(x[r], kb[pos], pr), m1 = cp_gimul(x[r], ka[pos], pr);
+      const cp_gi m0 = split_mul(x[r], kb[pos], pr), m1 = split_mul(x[r], ka[pos], pr);
       if (i == 0) {
         acc[r] = m0;
         if (active) park[r * TT + tk] = m1;
@@ -203,15 +229,16 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 #pragma unroll
       for (int r = 0; r < EPT; r++) acc[r] = park[r * TT + tl];
     }
-    dit_passes<LOGN, LOGN, 0, LOGN, 0>(acc, lds, t, twi, p, id);
+    gsi_passes<LOGN, LOGN, 0, LOGN, 0>(acc, lds, t, root, p, id);
     const int ts = cp_opaque(t);
+    const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
     if (active) {
       const cp_gi *ym = y + ((size_t)m * k + j) * N;
       global_gi *om = o + ((size_t)m * k + j) * N;
 #pragma unroll
       for (int r = 0; r < EPT; r++) {
         const int g = first_index<LOGN, 0, LOGN>(ts, r);
-        twist(acc[r], tsi, g, p);
+        split_back(acc[r], h, hp, hj, hjp, p);
         const cp_gi v = ym[g];
         om[g] = u64x2{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)};
         __builtin_amdgcn_sched_barrier(0);
@@ -222,14 +249,14 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
 
 template <int LOGN>
 static int launch_relin_fti(cupoly_ctx *c, cp_rcts out, const cp_gi *T, const cp_gi *key,
-                            cp_gi *park, int batch) {
+                            cp_gi *park, int batch) {  // key: the split-basis copy
   constexpr int R = rows_for<LOGN>(), THREADS = ((1 << LOGN) / EPT) * R;
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   cupoly_engine *e = c->e;
   const int nunits = batch * e->k;
   hipLaunchKernelGGL((k_relin_fti<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
-                     c->stream, out, T, key, park, batch, e->d_hps, e->d_tw_fwd, e->d_tw_inv,
-                     e->d_twist_fwd, e->d_twist_inv);
+                     c->stream, out, T, key, park, batch, e->d_hps, e->d_tw_fwd,
+                     e->d_split_root, e->d_split_tw, e->d_split_k);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }
@@ -289,10 +316,10 @@ static int relin_run(cupoly_ctx *c, const cupoly_relin_key *key, cp_rcts out, co
   if (relin_uses_fused(e, batch)) {
     cp_gi *park = (cp_gi *)ws;
     switch (e->logng) {
-      case 10: return launch_relin_fti<10>(c, out, T, key->d, park, batch);
-      case 11: return launch_relin_fti<11>(c, out, T, key->d, park, batch);
-      case 12: return launch_relin_fti<12>(c, out, T, key->d, park, batch);
-      default: return launch_relin_fti<13>(c, out, T, key->d, park, batch);
+      case 10: return launch_relin_fti<10>(c, out, T, key->ds, park, batch);
+      case 11: return launch_relin_fti<11>(c, out, T, key->ds, park, batch);
+      case 12: return launch_relin_fti<12>(c, out, T, key->ds, park, batch);
+      default: return launch_relin_fti<13>(c, out, T, key->ds, park, batch);
     }
   }
   cp_gi *Dg = (cp_gi *)ws, *ACC = Dg + (size_t)batch * k * k * ng;
@@ -352,9 +379,16 @@ extern "C" int cupoly_relin_key_create(cupoly_ctx *c, const cupoly_gi *s, uint64
       rc = cupoly_sample(c, (cupoly_gi *)(ee + (size_t)i * poly), CUPOLY_DISCRETE_GAUSSIAN, seed,
                          stream0 + 2 * (uint64_t)i + 1, CUPOLY_QBASE);
   }
+  if (!rc && relin_fused_sizes(e)) {
+    err = hipMalloc((void **)&key->ds, 2 * k * poly * sizeof(cp_gi));
+    if (err != hipSuccess) rc = cp_check_hip(err, "relin_key_create: hipMalloc");
+  }
   if (!rc) {
     hipLaunchKernelGGL(k_relin_key_b, dim3((ng + 255) / 256, k * k), dim3(256), 0, c->stream,
                        key->d, ee, sh, s2, ng, e->d_hps);
+    if (key->ds)
+      hipLaunchKernelGGL(k_relin_key_split, dim3((ng + 255) / 256, 2 * k * k), dim3(256), 0,
+                         c->stream, key->ds, key->d, ng, e->d_hps, e->d_split_k);
     err = hipGetLastError();
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
     if (err != hipSuccess) rc = cp_check_hip(err, "relin_key_create: key kernel");
@@ -364,6 +398,7 @@ extern "C" int cupoly_relin_key_create(cupoly_ctx *c, const cupoly_gi *s, uint64
   if (tmp) (void)hipFree(tmp);
   if (rc) {
     if (key->d) (void)hipFree(key->d);
+    if (key->ds) (void)hipFree(key->ds);
     delete key;
     return rc;
   }
@@ -373,7 +408,9 @@ extern "C" int cupoly_relin_key_create(cupoly_ctx *c, const cupoly_gi *s, uint64
 
 extern "C" int cupoly_relin_key_destroy(cupoly_relin_key *key) {
   if (!key) return CUPOLY_OK;
-  const hipError_t err = key->d ? hipFree(key->d) : hipSuccess;
+  hipError_t err = key->d ? hipFree(key->d) : hipSuccess;
+  const hipError_t err2 = key->ds ? hipFree(key->ds) : hipSuccess;
+  if (err == hipSuccess) err = err2;
   delete key;
   return err == hipSuccess ? CUPOLY_OK : cp_check_hip(err, "relin_key_destroy");
 }
