@@ -241,11 +241,22 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
   const uint64_t *twf = twf_all + (size_t)rid * N;
-  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4);
-  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const uint64_t *rootp = root_all + (size_t)rid * N * 4, *stwp = stw_all + (size_t)rid * N * 4;
   const uint64_t *sk = sk_all + (size_t)rid * 8;
+  if (ROWS == 1) {
+    rootp = uniform_ptr(rootp);
+    stwp = uniform_ptr(stwp);
+    a = uniform_ptr(a + (size_t)unit * N);
+    b = uniform_ptr(b + (size_t)unit * N);
+    out = uniform_ptr(out + (size_t)unit * N);
+  } else {
+    a += (size_t)unit * N;
+    b += (size_t)unit * N;
+    out += (size_t)unit * N;
+  }
+  const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)rootp;
+  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)stwp;
   const uint64_t J = sk[0], Jp = sk[1];
-  const size_t off = (size_t)unit * N;
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
   // a: load, convert and twist, transform; b is loaded only afterwards so
@@ -253,7 +264,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   // scheduling barriers keep the twist-table loads of one slot pair from being
   // hoisted above the previous pair's products (which would spill).
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = a[off + first_index<LOGN, 0, LOGN>(t, i)];
+  for (int i = 0; i < EPT; i++) xa[i] = a[first_index<LOGN, 0, LOGN>(t, i)];
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xa[i], xa[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
@@ -267,7 +278,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   int tb = t;
   asm volatile("" : "+v"(tb));
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xb[i] = b[off + first_index<LOGN, 0, LOGN>(tb, i)];
+  for (int i = 0; i < EPT; i++) xb[i] = b[first_index<LOGN, 0, LOGN>(tb, i)];
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
@@ -286,7 +297,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(to, i);
       split_back(xa[i], h, hp, hj, hjp, p);
-      out[off + g] = xa[i];
+      out[g] = xa[i];
       __builtin_amdgcn_sched_barrier(0);
     }
   }

@@ -66,6 +66,21 @@ __device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw,
   wp = v.y;
 }
 
+// Wave-uniform pointer: kept in SGPRs (the compiler otherwise holds some
+// uniform bases in VGPR pairs across the transforms, where they spill).  The
+// rebuilt address is first typed as a global-memory pointer, so the accesses
+// through it stay global loads and stores: a plain integer-to-pointer cast
+// gives a generic pointer, whose flat accesses also count against the LDS
+// wait counter and so stall the exchanges' LDS waits.
+template <class P>
+__device__ __forceinline__ P *uniform_ptr(P *q) {
+  const uint64_t v = (uint64_t)q;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  typedef __attribute__((address_space(1))) P global_p;
+  return (P *)(global_p *)(((uint64_t)hi << 32) | lo);
+}
+
 // One forward pass over local stages [LS0, LS0+R) of a 2^LOGM local array.
 // Global stage = local stage + SOFF; LOGN = log2 of the global transform size.
 // Butterflies whose twiddle is 1 are done without a multiply: always in the

@@ -204,20 +204,6 @@ __global__ void __launch_bounds__(256)
 // no store can overtake a park read.  Workgroups run residue-major (all pairs
 // of residue 0, then residue 1, ...), so consecutive workgroups share one
 // residue's twiddle and twist tables in L2.
-// Wave-uniform pointer: kept in SGPRs (the compiler otherwise holds some
-// uniform bases in VGPR pairs across the transforms, where they spill).  The
-// rebuilt address is first typed as a global-memory pointer, so the accesses
-// through it stay global loads and stores: a plain integer-to-pointer cast
-// gives a generic pointer, whose flat accesses also count against the LDS
-// wait counter and so stall the exchanges' LDS waits.
-template <class P>
-__device__ __forceinline__ P *uniform_ptr(P *q) {
-  const uint64_t v = (uint64_t)q;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  typedef __attribute__((address_space(1))) P global_p;
-  return (P *)(global_p *)(((uint64_t)hi << 32) | lo);
-}
 // Loads and stores of the seven-transform kernel, in the split basis of
 // dgt_dev.hpp: the spectra never leave the workgroup except through its own
 // parks, so the basis does not show in Y.
