@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
+  for (int i = 0; i < EPT; i++) xa[i] = split_mul_paired(xa[i], xb[i], pr);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
   int to = t;
   asm volatile("" : "+v"(to));

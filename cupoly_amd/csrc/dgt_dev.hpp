@@ -524,9 +524,17 @@ __device__ __forceinline__ void split_twist1(cp_gi &x, const CP_GLOBAL uint64_t 
   const cp_u64x2 kp = k[0], km = k[1];
   cp_shoup2v(x.re + t, kp.x, kp.y, x.re + p - t, km.x, km.y, p, x.re, x.imag);
 }
-// Slot-wise product in the split basis.
+// Slot-wise product in the split basis.  split_mul_paired forms the two
+// products with cp_mulmod2 (fewer instructions, more registers live at once:
+// used where the registers allow it).
 __device__ __forceinline__ cp_gi split_mul(const cp_gi &a, const cp_gi &b, const cp_prime &pr) {
   return cp_gi{cp_mulmod(a.re, b.re, pr), cp_mulmod(a.imag, b.imag, pr)};
+}
+__device__ __forceinline__ cp_gi split_mul_paired(const cp_gi &a, const cp_gi &b,
+                                                  const cp_prime &pr) {
+  cp_gi r;
+  cp_mulmod2(a.re, b.re, a.imag, b.imag, pr, r.re, r.imag);
+  return r;
 }
 // (U, V) -> (c_g, c_{g+N}) = ((U + V) h, (U - V) hj), h = 1/2N, hj = 1/2JN.
 __device__ __forceinline__ void split_back(cp_gi &x, uint64_t h, uint64_t hp, uint64_t hj,

@@ -182,6 +182,62 @@ CP_HD uint64_t cp_mulmod(uint64_t a, uint64_t b, const cp_prime& pr) {
   return cp_red2(z, p);
 }
 
+// The 127-bit product's pseudo-Mersenne reduction with its words given:
+// lo + hi 2^64 = a b < 2^127.  The folds are written on 32-bit words so that
+// each is one multiply-add: A = H0 delta + L (< 2^64), B = H1 delta + A.hi,
+// y = A.lo + B 2^32; then z = (y mod 2^63) + (y >> 63) delta.
+CP_HD uint64_t cp_fold127(uint64_t lo, uint64_t hi, uint32_t delta, uint64_t p) {
+  const uint64_t L = lo & ((1ULL << 63) - 1);
+  const uint64_t H = (hi << 1) | (lo >> 63);
+  const uint64_t A = (uint64_t)(uint32_t)H * delta + L;
+  const uint64_t B = (uint64_t)(uint32_t)(H >> 32) * delta + (A >> 32);
+  const uint32_t H2 = (uint32_t)(B >> 31);  // y >> 63, B < 2^62 + 2^32
+  const uint64_t L2 = ((uint64_t)((uint32_t)B & 0x7fffffffu) << 32) | (uint32_t)A;
+  return cp_red2(L2 + (uint64_t)H2 * delta, p);
+}
+
+// Two products of variables, rx = x y mod p and ru = u v mod p, each
+// product < 2^127 (one operand below 2^63).  The 128-bit products take the
+// carry-out of the middle column's second multiply-add (as cp_mulhi2v), then
+// cp_fold127.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void cp_mulmod2(uint64_t x, uint64_t y, uint64_t u, uint64_t v,
+                                           const cp_prime &pr, uint64_t &rx, uint64_t &ru) {
+  const uint64_t p = pr.p;
+  const uint32_t delta = (uint32_t)((1ULL << 63) - p);
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+  const uint32_t u0 = (uint32_t)u, u1 = (uint32_t)(u >> 32), v0 = (uint32_t)v, v1 = (uint32_t)(v >> 32);
+  const uint64_t px = (uint64_t)x0 * y0, pu = (uint64_t)u0 * v0;
+  const uint64_t mx = (uint64_t)x0 * y1 + (px >> 32);  // < 2^64 (one operand < 2^63)
+  const uint64_t mu = (uint64_t)u0 * v1 + (pu >> 32);
+  uint64_t sx, su, cx, cu;  // sx + cx 2^64 = x1 y0 + mx
+  asm("v_mad_u64_u32 %0, %2, %4, %5, %6\n\t"
+      "v_mad_u64_u32 %1, %3, %7, %8, %9"
+      : "=&v"(sx), "=&v"(su), "=&s"(cx), "=&s"(cu)
+      : "v"(x1), "v"(y0), "v"(mx), "v"(u1), "v"(v0), "v"(mu));
+  const uint64_t tx = (uint64_t)x1 * y1 + (sx >> 32);  // + cx 2^32 below: < 2^63 + 2^33
+  const uint64_t tu = (uint64_t)u1 * v1 + (su >> 32);
+  uint32_t hx, hu;
+  uint64_t jx, ju;
+  asm("v_addc_co_u32_e64 %0, %2, %4, 0, %5\n\t"
+      "v_addc_co_u32_e64 %1, %3, %6, 0, %7"
+      : "=&v"(hx), "=&v"(hu), "=&s"(jx), "=&s"(ju)
+      : "v"((uint32_t)(tx >> 32)), "s"(cx), "v"((uint32_t)(tu >> 32)), "s"(cu));
+  rx = cp_fold127(((uint64_t)(uint32_t)sx << 32) | (uint32_t)px,
+                  ((uint64_t)hx << 32) | (uint32_t)tx, delta, p);
+  ru = cp_fold127(((uint64_t)(uint32_t)su << 32) | (uint32_t)pu,
+                  ((uint64_t)hu << 32) | (uint32_t)tu, delta, p);
+}
+#else
+CP_HD void cp_mulmod2(uint64_t x, uint64_t y, uint64_t u, uint64_t v, const cp_prime &pr,
+                      uint64_t &rx, uint64_t &ru) {
+  const uint32_t delta = (uint32_t)((1ULL << 63) - pr.p);
+  const cp_u128 a = (cp_u128)x * y, b = (cp_u128)u * v;
+  rx = cp_fold127((uint64_t)a, (uint64_t)(a >> 64), delta, pr.p);
+  ru = cp_fold127((uint64_t)b, (uint64_t)(b >> 64), delta, pr.p);
+}
+#endif
+
 // Exact sums of products of 64-bit words, reduced mod p once at the end
 // (p = 2^63 - delta, delta < 2^30).  The sum is kept in 160 bits,
 // x = top 2^128 + hi 2^64 + lo (up to 2^32 products), and folded with

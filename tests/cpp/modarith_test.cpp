@@ -1,5 +1,7 @@
 // CPU check of the modular primitives in cupoly_amd/csrc/modarith.hpp (the
-// same inline functions the gfx950 kernels use) against unsigned __int128 %.
+// same inline functions the gfx950 kernels use, in their host forms: the
+// carry-chained asm of the paired device forms is checked on the GPU by the
+// parity tests) against unsigned __int128 %.
 // usage: modarith_test <primes...>; prints "modarith ok" or the first failure.
 #define CP_HD static inline
 #include <cstdio>
@@ -47,6 +49,25 @@ int main(int argc, char **argv) {
       if (cp_mulmod(a3, b3, pr) != (uint64_t)(((cp_u128)a3 * b3) % p)) {
         std::printf("mulmod (b < 2^64) FAIL p=%llu a=%llu b=%llu\n", (unsigned long long)p,
                     (unsigned long long)a3, (unsigned long long)b3);
+        return 1;
+      }
+      // the paired forms the kernels use: two products of variables
+      // (cp_fold127 on the 128-bit words), the column form of the 64-bit high
+      // product, and two Shoup products
+      uint64_t m1, m2;
+      cp_mulmod2(a, b, a3, b3, pr, m1, m2);
+      if (m1 != want2 || m2 != (uint64_t)(((cp_u128)a3 * b3) % p)) {
+        std::printf("mulmod2 FAIL p=%llu\n", (unsigned long long)p);
+        return 1;
+      }
+      if (cp_mulhi_col(a, wp) != cp_mulhi(a, wp) || cp_mulhi_col(b3, a) != cp_mulhi(b3, a)) {
+        std::printf("mulhi_col FAIL a=%llu\n", (unsigned long long)a);
+        return 1;
+      }
+      uint64_t s1, s2;
+      cp_shoup2(a, w, wp, b3, w, wp, p, s1, s2);
+      if (s1 != want || s2 != (uint64_t)(((cp_u128)b3 * w) % p)) {
+        std::printf("shoup2 FAIL p=%llu\n", (unsigned long long)p);
         return 1;
       }
       const uint64_t r = next64() % (2 * p);
