@@ -355,6 +355,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 __device__ __forceinline__ bool wave_local(int lst) { return lst <= 6; }
+// The barrier before an exchange's stores can also go when the previous
+// exchange of the same pass sequence used the same LDS map: each thread then
+// stores its registers, in the layout it loaded them in, to exactly the LDS
+// words it read itself, so no other wave can still be reading a word it
+// overwrites (and one wave's LDS accesses run in program order).  For
+// N = 8192 slots that holds for the exchange after the partial pass of a
+// forward sequence and the one into it of an inverse sequence: both use the
+// unpadded map of their neighbour.
+__device__ __forceinline__ bool same_map(LdsMap a, LdsMap b) { return a.A == b.A && a.S == b.S; }
 
 // Pass plan over local stages [LSB, LSE): an optional partial pass of
 // R0 = NS mod 3 stages first, then NF full radix-8 passes.  The full passes
@@ -389,6 +398,7 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);  // partial <-> first full
   t = cp_opaque(t);
+  LdsMap prev_m{-1, -1};  // map of the previous exchange of this sequence (none yet)
   if constexpr (PP::R0 > 0) {
     dif_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tw, p, map);
     if constexpr (PP::NF == 0) return;
@@ -396,6 +406,7 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     lds_store<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
     __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, PP::LSF, MP);
+    prev_m = MP;
   }
   bool prev_local = false, prev_pad = false;  // previous exchange of this loop
 #pragma nounroll
@@ -404,12 +415,13 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
     const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
     dif_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
-    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
+    if ((local && prev_local && pad == prev_pad) || same_map(m, prev_m)) wave_sync(); else __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0, m);
     if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     prev_local = local;
     prev_pad = pad;
+    prev_m = m;
   }
   if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 }
@@ -425,22 +437,24 @@ __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   t = cp_opaque(t);
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
   bool prev_local = false, prev_pad = false;
+  LdsMap prev_m{-1, -1};  // map of the previous exchange of this sequence (none yet)
 #pragma nounroll
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
     const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
-    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
+    if ((local && prev_local && pad == prev_pad) || same_map(m, prev_m)) wave_sync(); else __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
     dit_pass_rt<LOGN, LOGM, SOFF>(x, t, tw, p, map, ls0);
     prev_local = local;
     prev_pad = pad;
+    prev_m = m;
   }
   if constexpr (PP::R0 > 0) {
     if constexpr (PP::NF > 0) {
-      __syncthreads();
+      if (same_map(MP, prev_m)) wave_sync(); else __syncthreads();
       lds_store_rt<LOGM>(lds, x, t, PP::LSF, MP);
       __syncthreads();
       lds_load<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
@@ -644,22 +658,24 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   t = cp_opaque(t);
   if constexpr (PP::NF > 0) gsi_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tg, p, map);
   bool prev_local = false, prev_pad = false;
+  LdsMap prev_m{-1, -1};  // map of the previous exchange of this sequence (none yet)
 #pragma nounroll
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
     const bool local = wave_local(LOGM - ls0 - LOG_EPT), pad = m.A != 0;
-    if (local && prev_local && pad == prev_pad) wave_sync(); else __syncthreads();
+    if ((local && prev_local && pad == prev_pad) || same_map(m, prev_m)) wave_sync(); else __syncthreads();
     lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
     gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0);
     prev_local = local;
     prev_pad = pad;
+    prev_m = m;
   }
   if constexpr (PP::R0 > 0) {
     if constexpr (PP::NF > 0) {
-      __syncthreads();
+      if (same_map(MP, prev_m)) wave_sync(); else __syncthreads();
       lds_store_rt<LOGM>(lds, x, t, PP::LSF, MP);
       __syncthreads();
       lds_load<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);

@@ -1,9 +1,12 @@
 """LDS exchange synchronisation of the DGT passes, checked on the CPU.
 
-dif_passes / dit_passes (cupoly_amd/csrc/dgt_dev.hpp) skip the workgroup
-barriers of an exchange that is wave-local (each wave reads back only the
-slots it wrote) and, when the previous exchange was also wave-local under a
-map with the same padding ratio, the barrier before its stores.  This test
+dif_passes / dit_passes / gsi_passes (cupoly_amd/csrc/dgt_dev.hpp; the last
+two share one exchange sequence) skip the workgroup barriers of an exchange
+that is wave-local (each wave reads back only the slots it wrote).  They skip
+the barrier before an exchange's stores when the previous exchange was also
+wave-local under a map with the same padding ratio, or when the previous
+exchange of the sequence used the same map (each thread then overwrites only
+the words it read itself).  This test
 replays the LDS addresses those functions compute (the same pass layouts,
 padding maps and skip rules) for every thread of every workgroup shape the
 kernels launch, and requires that any two accesses to one LDS slot from
@@ -57,7 +60,7 @@ def plan(LSB, LSE):
     return R0, NF, LSB + R0
 
 
-def dif_ops(L, LSB, LSE, local_max=6):
+def dif_ops(L, LSB, LSE, local_max=6, same_map=True):
     R0, NF, LSF = plan(LSB, LSE)
     MP = lds_map_for(L - LSF - LOG_EPT)
     ops = []
@@ -67,34 +70,39 @@ def dif_ops(L, LSB, LSE, local_max=6):
         ops += [("sync",), ("st", lambda t: fixed_addrs(L, LSB, R0, MP, t)), ("sync",),
                 ("ld", lambda t, ls=LSF: rt_addrs(L, t, ls, MP))]
     prev_local = prev_pad = False
+    prev_m = MP if R0 > 0 else None
     for i in range(NF - 1):
         ls0 = LSF + LOG_EPT * i
         m = lds_map_for(L - ls0 - 2 * LOG_EPT)
         local = (L - ls0 - LOG_EPT) <= local_max; pad = m[0] != 0
-        ops.append(("wave",) if (local and prev_local and pad == prev_pad) else ("sync",))
+        skip = (local and prev_local and pad == prev_pad) or (same_map and m == prev_m)
+        ops.append(("wave",) if skip else ("sync",))
         ops.append(("st", lambda t, ls=ls0, mm=m: rt_addrs(L, t, ls, mm)))
         ops.append(("wave",) if local else ("sync",))
         ops.append(("ld", lambda t, ls=ls0 + LOG_EPT, mm=m: rt_addrs(L, t, ls, mm)))
-        prev_local, prev_pad = local, pad
+        prev_local, prev_pad, prev_m = local, pad, m
     return ops
 
 
-def dit_ops(L, LSB, LSE, local_max=6):
+def dit_ops(L, LSB, LSE, local_max=6, same_map=True):
     R0, NF, LSF = plan(LSB, LSE)
     MP = lds_map_for(L - LSF - LOG_EPT)
     ops = []
     prev_local = prev_pad = False
+    prev_m = None
     for i in range(NF - 2, -1, -1):
         ls0 = LSF + LOG_EPT * i
         m = lds_map_for(L - ls0 - 2 * LOG_EPT)
         local = (L - ls0 - LOG_EPT) <= local_max; pad = m[0] != 0
-        ops.append(("wave",) if (local and prev_local and pad == prev_pad) else ("sync",))
+        skip = (local and prev_local and pad == prev_pad) or (same_map and m == prev_m)
+        ops.append(("wave",) if skip else ("sync",))
         ops.append(("st", lambda t, ls=ls0 + LOG_EPT, mm=m: rt_addrs(L, t, ls, mm)))
         ops.append(("wave",) if local else ("sync",))
         ops.append(("ld", lambda t, ls=ls0, mm=m: rt_addrs(L, t, ls, mm)))
-        prev_local, prev_pad = local, pad
+        prev_local, prev_pad, prev_m = local, pad, m
     if R0 > 0 and NF > 0:
-        ops += [("sync",), ("st", lambda t: rt_addrs(L, t, LSF, MP)), ("sync",),
+        ops += [("wave",) if (same_map and MP == prev_m) else ("sync",),
+                ("st", lambda t: rt_addrs(L, t, LSF, MP)), ("sync",),
                 ("ld", lambda t: fixed_addrs(L, LSB, R0, MP, t))]
     return ops
 
@@ -155,9 +163,37 @@ def test_exchanges_are_race_free(L, lsb, lse, rows):
 
 
 def test_wave_local_exchanges_exist_at_n16384():
-    # the two exchanges closest to the last stages skip barriers at N_g = 8192
-    _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13), dit_ops(13, 0, 13)])
+    # the two exchanges closest to the last stages skip barriers at N_g = 8192,
+    # and the exchange after the partial pass (forward) and the one into it
+    # (inverse) skip the barrier before their stores (same unpadded map as
+    # their neighbour)
+    _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13, same_map=False),
+                                     dit_ops(13, 0, 13, same_map=False)])
     assert nwave == 6
+    _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13), dit_ops(13, 0, 13)])
+    assert nwave == 8
+
+
+def test_planted_map_mismatch_skip_is_reported():
+    """Skipping the barrier before the stores is safe only when the previous
+    exchange used the same map: pretend a padded map is the same."""
+    L = 13
+    R0, NF, LSF = plan(0, L)
+
+    def bad_dif():
+        ops = []
+        prev_m = None
+        for i in range(NF - 1):
+            ls0 = LSF + LOG_EPT * i
+            m = (1, 3) if i % 2 else (0, 0)  # alternate maps, skip regardless
+            ops.append(("wave",) if prev_m is not None else ("sync",))
+            ops.append(("st", lambda t, ls=ls0, mm=m: rt_addrs(L, t, ls, mm)))
+            ops.append(("sync",))
+            ops.append(("ld", lambda t, ls=ls0 + LOG_EPT, mm=m: rt_addrs(L, t, ls, mm)))
+            prev_m = m
+        return ops
+    bad, _ = check(L, 0, L, rows_for(L), [bad_dif()])
+    assert bad > 0
 
 
 def test_planted_cross_wave_skip_is_reported():
