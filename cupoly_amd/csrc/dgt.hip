@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
   for (int i = 0; i < EPT; i++) xa[i] = split_mul_paired(xa[i], xb[i], pr);
-  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
   int to = t;
   asm volatile("" : "+v"(to));
   const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];

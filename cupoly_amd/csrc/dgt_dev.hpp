@@ -196,17 +196,21 @@ __device__ __forceinline__ int pass_index_rt(int t, int i, int ls0) {
 // consecutive slots per lane) or ST = 8 (runs of 8 lanes 64 slots apart) hit
 // only 2 of them.  Each exchange therefore pads the array linearly,
 //     physical = l + A * (l >> S),
-// with (A, S) = (1, 3) when the smaller stride of its two layouts is 1,
-// (8, 6) when it is 8, and no padding otherwise.  Both maps are
-// conflict-free for both layouts of their exchange (and for the natural
+// with (A, S) = (1, 3) when the smaller stride of its two layouts is 1, and
+// (8, 6) when it is 8, 64, 512 or 4096.  (8, 6) is needed at stride 8; at
+// the larger strides 16 lanes touch 16 consecutive slots inside one 64-slot
+// run, which it leaves consecutive, so it costs nothing there and lets the
+// exchanges of a pass sequence share one map (below: the barrier before the
+// stores can then go).  Other strides (16, 32) take no padding.  The maps
+// are conflict-free for both layouts of their exchange (and for the natural
 // order used to stage global I/O), and keep every address a per-thread base
-// plus i times a uniform stride.  A 2^LOGM-slot array then spans
-// 2^LOGM + 2^LOGM / 8 slots.
+// plus i times a uniform stride (S <= the stride's log2, or S = it + 3).  A
+// 2^LOGM-slot array then spans 2^LOGM + 2^LOGM / 8 slots.
 struct LdsMap {
   int A, S;
 };
 __host__ __device__ constexpr LdsMap lds_map_for(int lst_lo) {
-  return lst_lo == 0 ? LdsMap{1, 3} : lst_lo == 3 ? LdsMap{8, 6} : LdsMap{0, 0};
+  return lst_lo == 0 ? LdsMap{1, 3} : (lst_lo >= 3 && lst_lo % 3 == 0) ? LdsMap{8, 6} : LdsMap{0, 0};
 }
 template <int LOGM>
 __host__ __device__ constexpr int lds_row() {
@@ -359,10 +363,9 @@ __device__ __forceinline__ bool wave_local(int lst) { return lst <= 6; }
 // exchange of the same pass sequence used the same LDS map: each thread then
 // stores its registers, in the layout it loaded them in, to exactly the LDS
 // words it read itself, so no other wave can still be reading a word it
-// overwrites (and one wave's LDS accesses run in program order).  For
-// N = 8192 slots that holds for the exchange after the partial pass of a
-// forward sequence and the one into it of an inverse sequence: both use the
-// unpadded map of their neighbour.
+// overwrites (and one wave's LDS accesses run in program order).  With the
+// maps above, every exchange of an N = 8192-slot sequence but the last
+// (stride 1) uses (8, 6).
 __device__ __forceinline__ bool same_map(LdsMap a, LdsMap b) { return a.A == b.A && a.S == b.S; }
 
 // Pass plan over local stages [LSB, LSE): an optional partial pass of
@@ -649,8 +652,12 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
 
 // Split-basis inverse passes over local stages [LSB, LSE) in reverse;
 // registers start in the last-pass layout and end in the first-pass layout
-// (the exchanges of dit_passes).
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW>
+// (the exchanges of dit_passes).  AFTER_DIF: the last LDS access of this
+// workgroup was the last exchange of dif_passes over the same stages, which
+// loaded the registers' layout with the map this sequence's first exchange
+// stores with, so the barrier before those stores can go as well.
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, bool AFTER_DIF = false, class Map,
+          class TW>
 __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            TW __restrict__ tg, uint64_t p, const Map &map) {
   using PP = PassPlan<LSB, LSE>;
@@ -658,7 +665,10 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   t = cp_opaque(t);
   if constexpr (PP::NF > 0) gsi_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tg, p, map);
   bool prev_local = false, prev_pad = false;
-  LdsMap prev_m{-1, -1};  // map of the previous exchange of this sequence (none yet)
+  // map of the previous exchange of this sequence (none yet)
+  LdsMap prev_m = AFTER_DIF && PP::NF >= 2
+                      ? lds_map_for(LOGM - (PP::LSF + LOG_EPT * (PP::NF - 2)) - 2 * LOG_EPT)
+                      : LdsMap{-1, -1};
 #pragma nounroll
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
@@ -805,7 +815,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
     park_ld(park, h, t, y);
 #pragma unroll
     for (int i = 0; i < EPT; i++) x[i] = split_mul(x[i], y[i], pr);
-    gsi_passes<14, 13, 0, 13, 1>(x, lds, t, root, p, MapBlock{h * H});
+    gsi_passes<14, 13, 0, 13, 1, true>(x, lds, t, root, p, MapBlock{h * H});
     if (h == 0) park_st(park, 0, t, x);
   }
 #pragma unroll

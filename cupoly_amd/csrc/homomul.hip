@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
 #pragma unroll
   for (int i = 0; i < EPT; i++) xb[i] = split_mul(xa[i], xb[i], pr);
-  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, root, p, id);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xb, lds, t, root, p, id);  // right after S_d0's passes
   fti_store<LOGN>(y0, xb, t, sk, p, active);
   fti_load<LOGN>(xb, d1, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d1, held
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     xa[i].imag = cp_add(w.imag, pc0d1.imag, p);
     __builtin_amdgcn_sched_barrier(0);
   }
-  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);  // right after S_c1's passes
   fti_store<LOGN>(y1, xa, t, sk, p, active);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, root, p, id);
   fti_store<LOGN>(y2, xb, t, sk, p, active);

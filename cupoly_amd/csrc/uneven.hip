@@ -151,7 +151,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
   for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
-  gsi_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, root, p, id);
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
   int to = t;
   asm volatile("" : "+v"(to));
   const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];

@@ -23,7 +23,7 @@ EPT, LOG_EPT = 8, 3
 
 
 def lds_map_for(lst_lo):
-    return (1, 3) if lst_lo == 0 else (8, 6) if lst_lo == 3 else (0, 0)
+    return (1, 3) if lst_lo == 0 else (8, 6) if (lst_lo >= 3 and lst_lo % 3 == 0) else (0, 0)
 
 
 def lds_row(L):
@@ -84,12 +84,14 @@ def dif_ops(L, LSB, LSE, local_max=6, same_map=True):
     return ops
 
 
-def dit_ops(L, LSB, LSE, local_max=6, same_map=True):
+def dit_ops(L, LSB, LSE, local_max=6, same_map=True, after_dif=False):
     R0, NF, LSF = plan(LSB, LSE)
     MP = lds_map_for(L - LSF - LOG_EPT)
     ops = []
     prev_local = prev_pad = False
-    prev_m = None
+    # after_dif: the previous LDS access was dif_ops' last exchange over the
+    # same stages (gsi_passes<..., AFTER_DIF = true>)
+    prev_m = lds_map_for(L - (LSF + LOG_EPT * (NF - 2)) - 2 * LOG_EPT) if after_dif and NF >= 2 else None
     for i in range(NF - 2, -1, -1):
         ls0 = LSF + LOG_EPT * i
         m = lds_map_for(L - ls0 - 2 * LOG_EPT)
@@ -107,11 +109,16 @@ def dit_ops(L, LSB, LSE, local_max=6, same_map=True):
     return ops
 
 
-def check(L, LSB, LSE, rows, seqs):
+def check(L, LSB, LSE, rows, seqs, joined=()):
+    """seqs run in order with a workgroup barrier between them, except before
+    the sequences whose index is in `joined` (run straight after the one
+    before, as gsi_passes<AFTER_DIF> after dif_passes)."""
     T = (1 << L) // EPT
     nthreads = T * rows
     ops = [("sync",)]
-    for s in seqs:
+    for k, s in enumerate(seqs):
+        if k in joined:
+            ops.pop()
         ops += s + [("sync",)]
     acc = collections.defaultdict(list)  # addr -> [(op index, wave, is_store)]
     span = lds_row(L)
@@ -160,18 +167,47 @@ def test_exchanges_are_race_free(L, lsb, lse, rows):
     seq = [dif_ops(L, lsb, lse), dif_ops(L, lsb, lse), dit_ops(L, lsb, lse)]
     bad, _ = check(L, lsb, lse, rows, seq)
     assert bad == 0
+    # the fused product: the inverse runs straight after b's forward sequence
+    seq = [dif_ops(L, lsb, lse), dif_ops(L, lsb, lse), dit_ops(L, lsb, lse, after_dif=True)]
+    bad, _ = check(L, lsb, lse, rows, seq, joined=(2,))
+    assert bad == 0
+
+
+def bank_conflicts(L, rows, ops):
+    """16-lane groups of 16-byte accesses; a group is conflict-free when its 16
+    slots are distinct modulo 16 (256 bytes).  Returns (conflicted, groups)."""
+    T = (1 << L) // EPT
+    nthreads = T * rows
+    bad = tot = 0
+    for op in ops:
+        if op[0] not in ("st", "ld"):
+            continue
+        for g0 in range(0, nthreads, 16):
+            tids = range(g0, min(g0 + 16, nthreads))
+            addrs = [op[1](tid % T) for tid in tids]
+            base = [(tid // T) * lds_row(L) for tid in tids]
+            for i in range(EPT):
+                tot += 1
+                if len({(b + a[i]) % 16 for b, a in zip(base, addrs)}) != len(addrs):
+                    bad += 1
+    return bad, tot
+
+
+@pytest.mark.parametrize("L,lsb,lse,rows", [c for c in CASES if c[0] >= 10 or c[3] > 1])
+def test_exchanges_are_bank_conflict_free(L, lsb, lse, rows):
+    bad, tot = bank_conflicts(L, rows, dif_ops(L, lsb, lse) + dit_ops(L, lsb, lse))
+    assert tot > 0 and bad == 0
 
 
 def test_wave_local_exchanges_exist_at_n16384():
     # the two exchanges closest to the last stages skip barriers at N_g = 8192,
-    # and the exchange after the partial pass (forward) and the one into it
-    # (inverse) skip the barrier before their stores (same unpadded map as
-    # their neighbour)
+    # and every exchange whose previous one in the sequence used the same map
+    # skips the barrier before its stores
     _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13, same_map=False),
                                      dit_ops(13, 0, 13, same_map=False)])
     assert nwave == 6
     _, nwave = check(13, 0, 13, 1, [dif_ops(13, 0, 13), dit_ops(13, 0, 13)])
-    assert nwave == 8
+    assert nwave == 10
 
 
 def test_planted_map_mismatch_skip_is_reported():
