@@ -42,6 +42,7 @@ SIGNATURES = {
     "cupoly_engine_primes": ([_vp, _vp], _int),
     "cupoly_engine_constant": ([_vp, ctypes.c_char_p, _vp, _int], _int),
     "cupoly_hps_constant": ([_int, _int, _u64, ctypes.c_char_p, _vp, _int], _int),
+    "cupoly_arith_selftest": ([_vp, _vp, _vp, _int, _u64], _int),
     "cupoly_ctx_create": ([_vp, _vp, ctypes.POINTER(_vp)], _int),
     "cupoly_ctx_destroy": ([_vp], _int),
     "cupoly_ctx_stream": ([_vp], _vp),
@@ -194,6 +195,18 @@ def hps_constant(k, kl, t, name):
     buf = (ctypes.c_uint64 * 512)()
     cnt = _check(lib().cupoly_hps_constant(k, kl, t, name.encode(), buf, 512))
     return [int(buf[i]) for i in range(cnt)]
+
+
+def arith_selftest(ctx, records, p):
+    """Run the device arithmetic self-test: records is an (n, 4) int64 device
+    tensor of (x, y, w, z) words; returns the (n, 12) device tensor of results
+    (csrc/selftest.hip lists them)."""
+    n = int(records.shape[0])
+    if records.dim() != 2 or records.shape[1] != 4:
+        raise ValueError("arith_selftest: records must have shape (n, 4)")
+    out = empty((n, 12))
+    _check(lib().cupoly_arith_selftest(ctx.handle, _ptr(records), _ptr(out), n, p))
+    return out
 
 
 class Context:

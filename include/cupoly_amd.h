@@ -84,6 +84,14 @@ int cupoly_engine_constant(const cupoly_engine *e, const char *name, uint64_t *o
  * RD(1/RN(q_j)). */
 int cupoly_hps_constant(int k, int kl, uint64_t t, const char *name, uint64_t *out, int cap);
 
+/* Device self-test of the arithmetic primitives whose gfx950 forms are inline
+ * asm (no reference counterpart: a diagnostic for tests).  in = n records of
+ * 4 words (x, y, w, z); out = n records of 12 words: high products, Shoup
+ * products, products of variables, 160-bit multiply-accumulate sums mod p and
+ * round-down FMAs (csrc/selftest.hip lists them).  p must lie in
+ * (2^63 - 2^30, 2^63).  Asynchronous on the context's stream. */
+int cupoly_arith_selftest(cupoly_ctx *c, const uint64_t *in, uint64_t *out, int n, uint64_t p);
+
 /* ---- context: Context() / ~Context(), context.cpp:19-67 ------------------
  * stream = 0 creates a private stream (as the reference); otherwise the
  * context uses the caller's hipStream_t. */
