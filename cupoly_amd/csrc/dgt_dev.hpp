@@ -721,11 +721,13 @@ constexpr int rows_for() {
 // P1 = a's half 1; P2 = b's half 1.
 __device__ __forceinline__ void park_st(CP_GLOBAL cp_gi *park, int area, int t,
                                         const cp_gi (&x)[EPT]) {
+  t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) st_gi(park + (area * EPT + i) * 1024 + t, x[i]);
 }
 __device__ __forceinline__ void park_ld(const CP_GLOBAL cp_gi *park, int area, int t,
                                         cp_gi (&x)[EPT]) {
+  t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) x[i] = ld_gi(park + (area * EPT + i) * 1024 + t);
 }
@@ -738,6 +740,7 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
                                            const CP_GLOBAL uint64_t *stw, uint64_t J, uint64_t Jp,
                                            uint64_t p) {
   constexpr int H = 8192;
+  t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
@@ -760,6 +763,7 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
                                             const CP_GLOBAL uint64_t *root,
                                             const uint64_t *sk, uint64_t p) {
   constexpr int H = 8192;
+  t = cp_opaque(t);
   uint64_t wu, wup, wv, wvp;
   ld_root2(root, 1u, wu, wup, wv, wvp);
   const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
@@ -804,9 +808,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   }
   // b: per half, spectrum, product with a's (P0 / P1), inverse stages 13..1;
   // half 0's result waits in P0
-  int tb = t;
-  asm volatile("" : "+v"(tb));
-  stage0_16k(x, y, b, tb, twf, stw, J, Jp, p);
+  stage0_16k(x, y, b, t, twf, stw, J, Jp, p);
   park_st(park, 2, t, y);
 #pragma nounroll
   for (int h = 0; h < 2; h++) {
@@ -821,7 +823,5 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
 #pragma unroll
   for (int i = 0; i < EPT; i++) y[i] = x[i];
   park_ld(park, 0, t, x);
-  int to = t;
-  asm volatile("" : "+v"(to));
-  istage0_16k(c, x, y, to, root, sk, p);
+  istage0_16k(c, x, y, t, root, sk, p);
 }

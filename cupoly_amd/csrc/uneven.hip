@@ -129,17 +129,17 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   CP_GLOBAL cp_gi *uc = glb(u.c);
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
+  const int ta = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
-    const int g0 = first_index<LOGN, 0, LOGN>(t, i), g1 = first_index<LOGN, 0, LOGN>(t, i + 1);
+    const int g0 = first_index<LOGN, 0, LOGN>(ta, i), g1 = first_index<LOGN, 0, LOGN>(ta, i + 1);
     xa[i] = ld_gi(ua + g0);
     xa[i + 1] = ld_gi(ua + g1);
     split_twist2(xa[i], xa[i + 1], stw, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
-  int tb = t;
-  asm volatile("" : "+v"(tb));
+  const int tb = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     const int g0 = first_index<LOGN, 0, LOGN>(tb, i), g1 = first_index<LOGN, 0, LOGN>(tb, i + 1);
@@ -152,8 +152,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
 #pragma unroll
   for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
-  int to = t;
-  asm volatile("" : "+v"(to));
+  const int to = cp_opaque(t);
   const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
   if (active) {
 #pragma unroll
