@@ -559,8 +559,9 @@ def hom_mul_gather(ctx, outs, ct1s, ct2s):
 
 def polymul_set_tuning(ctx, rows=0, persistent=0):
     """Launch choices of polymul_batch on ctx: rows per workgroup (n = 2^11 ..
-    2^13) and, for n = 2^15, persistent (1) or two-kernel split (-1); 0 keeps
-    the built-in choice."""
+    2^13) and persistent: for n = 2^11 .. 2^14 with one row per workgroup the
+    streaming form (1) or one-shot workgroups (-1), for n = 2^15 persistent (1)
+    or two-kernel split (-1); 0 keeps the built-in choice."""
     _check(lib().cupoly_polymul_set_tuning(ctx.handle, int(rows), int(persistent)))
 
 

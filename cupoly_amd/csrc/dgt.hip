@@ -365,6 +365,97 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// Persistent form of the batched product with one residue polynomial per
+// workgroup (n/2 = 2^10 .. 2^13 slots).  The grid holds as many workgroups as
+// the GPU runs at once; each walks the units blockIdx.x, + gridDim.x, ...
+// Operand loads start while the previous transform runs its last register
+// pass: b's straight into registers during a's last forward pass; the next
+// unit's a as an LDS copy (lds_copy_issue: no registers are free there)
+// during the last inverse pass, the conversion back and the stores of this
+// unit.  One-shot workgroups all load their operands at the same moments and
+// leave the CUs waiting on bursts of HBM reads.  Results are the same as
+// k_polymul_fused<LOGN, 1> (same passes, same arithmetic).
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / EPT)
+    k_polymul_stream(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
+                     const cp_gi *__restrict__ b, int nunits, int nres,
+                     const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
+                     const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
+                     const uint64_t *__restrict__ sk_all) {
+  constexpr int N = 1 << LOGN, T = N / EPT;
+  static_assert(T % 64 == 0 && PassPlan<0, LOGN>::NF > 0, "whole waves, a register last pass");
+  static_assert(T * EPT <= lds_row<LOGN>(), "the copy areas fit the exchange array");
+  extern __shared__ cp_gi lds[];
+  const int t = (int)threadIdx.x, lane = t & 63;
+  cp_gi *area = lds + __builtin_amdgcn_readfirstlane(t >> 6) * (EPT * 64);
+  MapIdentity id;
+  int unit = blockIdx.x;
+  if (unit >= nunits) return;  // uniform
+  lds_copy_issue<LOGN>(area, (const CP_GLOBAL cp_gi *)uniform_ptr(a + (size_t)unit * N), t);
+  for (; unit < nunits; unit += gridDim.x) {  // uniform trip count per workgroup
+    const int rid = __builtin_amdgcn_readfirstlane(unit % nres);
+    const cp_prime pr = primes[rid];
+    const uint64_t p = pr.p;
+    const uint64_t *twf = twf_all + (size_t)rid * N;
+    const CP_GLOBAL uint64_t *root =
+        (const CP_GLOBAL uint64_t *)uniform_ptr(root_all + (size_t)rid * N * 4);
+    const CP_GLOBAL uint64_t *stw =
+        (const CP_GLOBAL uint64_t *)uniform_ptr(stw_all + (size_t)rid * N * 4);
+    const uint64_t *sk = sk_all + (size_t)rid * 8;
+    const uint64_t J = sk[0], Jp = sk[1];
+    const int next = unit + (int)gridDim.x;
+    cp_gi xa[EPT], xb[EPT];
+    // a: from its LDS copy; convert and twist; transform.  b is loaded into
+    // registers at the start of a's last forward pass (which has no other
+    // vector-memory loads: its twiddles are uniform, scalar loads).
+    lds_copy_wait();
+    lds_copy_read(xa, area, lane);
+    {
+      const int ta = cp_opaque(t);
+#pragma unroll
+      for (int i = 0; i < EPT; i += 2) {
+        split_twist2(xa[i], xa[i + 1], stw, first_index<LOGN, 0, LOGN>(ta, i),
+                     first_index<LOGN, 0, LOGN>(ta, i + 1), J, Jp, p);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    const CP_GLOBAL cp_gi *ub = (const CP_GLOBAL cp_gi *)uniform_ptr(b + (size_t)unit * N);
+    dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id, [&] {
+      const int tq = cp_opaque(t);
+#pragma unroll
+      for (int i = 0; i < EPT; i++) xb[i] = ld_gi(ub + first_index<LOGN, 0, LOGN>(tq, i));
+    });
+    pin_regs(xa);
+    {
+      const int tb = cp_opaque(t);
+#pragma unroll
+      for (int i = 0; i < EPT; i += 2) {
+        split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
+                     first_index<LOGN, 0, LOGN>(tb, i + 1), J, Jp, p);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) xa[i] = split_mul_paired(xa[i], xb[i], pr);
+    gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id, [&] {
+      __syncthreads();
+      if (next < nunits)
+        lds_copy_issue<LOGN>(area, (const CP_GLOBAL cp_gi *)uniform_ptr(a + (size_t)next * N),
+                             t);
+    });
+    const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
+    CP_GLOBAL cp_gi *uo = (CP_GLOBAL cp_gi *)uniform_ptr(out + (size_t)unit * N);
+    const int to = cp_opaque(t);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      split_back(xa[i], h, hp, hj, hjp, p);
+      st_gi(uo + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 // -------------------------------------------------------------------------
 // Host side
 // -------------------------------------------------------------------------
@@ -394,6 +485,8 @@ static int setup_size() {
       if ((rc = set_lds(k_polymul_fused<LOGN, 4>, 4 * lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
     if constexpr (LOGN == 10)
       if ((rc = set_lds(k_polymul_fused<LOGN, 8>, 8 * lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
+    if constexpr (LOGN >= 10)
+      if ((rc = set_lds(k_polymul_stream<LOGN>, lds_row<LOGN>() * sizeof(cp_gi)))) return rc;
   } else {
     const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     if ((rc = set_lds(k_dgt_cols<LOGN, true>, lds))) return rc;
@@ -513,6 +606,26 @@ static void launch_fused(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi 
                      e->d_split_root, e->d_split_tw, e->d_split_k);
 }
 
+// Persistent (streaming) form: as many workgroups as the GPU holds at once.
+template <int LOGN>
+static int launch_stream(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, int nunits) {
+  constexpr int T = (1 << LOGN) / EPT;
+  cupoly_engine *e = c->e;
+  const size_t lds = (size_t)lds_row<LOGN>() * sizeof(cp_gi);
+  static int per_cu = 0;  // the same for every gfx950 device
+  if (per_cu == 0) {
+    int n = 0;
+    CP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_polymul_stream<LOGN>, T, lds));
+    per_cu = n > 0 ? n : 1;
+  }
+  const long slots = (long)per_cu * e->ncu;
+  const int grid = nunits < slots ? nunits : (int)slots;
+  hipLaunchKernelGGL(k_polymul_stream<LOGN>, dim3(grid), dim3(T), lds, c->stream, out, a, b,
+                     nunits, e->k, e->d_primes, e->d_tw_fwd, e->d_split_root, e->d_split_tw,
+                     e->d_split_k);
+  return CUPOLY_OK;
+}
+
 // Rows (residue polynomials) per workgroup the batched product can use at
 // this size besides the built-in rows_for: n = 2^11 .. 2^13 only.
 static bool fused_rows_ok(int logng, int rows) {
@@ -531,7 +644,25 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
   const bool persistent =
       c->tune_persistent ? c->tune_persistent > 0 : 8 * nunits >= 5 * e->ncu;
   if constexpr (LOGN <= 13) {
-    const int rows = fused_rows_ok(LOGN, c->tune_rows) ? c->tune_rows : rows_for<LOGN>();
+    // one residue polynomial per workgroup by default from n = 2^11 (at
+    // n = 2^11, k = 4, batch 8192: 1.03 ms against 1.08 ms with two)
+    const int rows = fused_rows_ok(LOGN, c->tune_rows) ? c->tune_rows
+                     : LOGN == 10                      ? 1
+                                                       : rows_for<LOGN>();
+    // the streaming form: by default at n = 2^14 once the batch has more
+    // residue products than CUs (k = 4, batch 1024: 1.342 against 1.384 ms
+    // one-shot); at n = 2^11 .. 2^13, where several one-shot workgroups share
+    // a CU, it is slower (1.08 / 1.14 / 1.22 against 1.03 / 1.10 / 1.18 ms)
+    // and runs only when asked for
+    if constexpr (LOGN >= 10) {
+      if (rows == 1 && c->tune_persistent >= 0 &&
+          (c->tune_persistent > 0 || (LOGN == 13 && nunits > e->ncu))) {
+        int rc = launch_stream<LOGN>(c, out, a, b, nunits);
+        if (rc) return rc;
+        CP_LAUNCH_CHECK();
+        return CUPOLY_OK;
+      }
+    }
     if constexpr (LOGN >= 10 && LOGN <= 12) {
       if (rows == 1) launch_fused<LOGN, 1>(c, out, a, b, nunits);
       else if (rows == 2) launch_fused<LOGN, 2>(c, out, a, b, nunits);
@@ -636,8 +767,9 @@ extern "C" int cupoly_polymul_set_tuning(cupoly_ctx *c, int rows, int persistent
                                        " not available at this ring degree");
   if (persistent < -1 || persistent > 1)
     return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: persistent must be -1, 0 or 1");
-  if (persistent != 0 && c->e->logng != 14)
-    return cp_fail(CUPOLY_ERR_ARG, "polymul_set_tuning: the persistent choice exists for n = 2^15 only");
+  if (persistent != 0 && (c->e->logng < 10 || c->e->logng > 14))
+    return cp_fail(CUPOLY_ERR_ARG,
+                   "polymul_set_tuning: the persistent choice exists for n = 2^11 .. 2^15 only");
   c->tune_rows = rows;
   c->tune_persistent = persistent;
   return CUPOLY_OK;
@@ -654,8 +786,9 @@ extern "C" int cupoly_polymul_autotune(cupoly_ctx *c, cupoly_gi *out, const cupo
   std::vector<std::pair<int, int>> cand;
   const int logng = c->e->logng;
   for (int r : {1, 2, 4, 8})
-    if (fused_rows_ok(logng, r)) cand.push_back({r, 0});
-  if (logng == 14) cand = {{0, 1}, {0, -1}};
+    if (fused_rows_ok(logng, r)) cand.push_back({r, r == 1 ? -1 : 0});
+  if (logng >= 10 && logng <= 12) cand.push_back({1, 1});
+  if (logng == 13 || logng == 14) cand = {{0, 1}, {0, -1}};
   if (cand.empty()) cand.push_back({0, 0});
   hipEvent_t e0, e1;
   CP_HIP(hipEventCreate(&e0));

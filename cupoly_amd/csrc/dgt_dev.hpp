@@ -392,12 +392,20 @@ __device__ __forceinline__ int first_index(int t, int i) {
   return pass_index<LOGM, PP::LS_FIRST, PP::R_FIRST>(t, i);
 }
 
+// Called by a pass sequence after its last LDS read, before its last pass
+// (which runs in registers): the LDS array is free from there on for the
+// workgroup's waves once they have all passed a barrier.
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 // Forward passes over local stages [LSB, LSE), registers starting in the
 // first-pass layout and ending in the last-pass layout.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW>
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, class Map, class TW,
+          class Hook = NoHook>
 __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            TW __restrict__ tw, uint64_t p,
-                                           const Map &map) {
+                                           const Map &map, const Hook &hook = Hook()) {
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);  // partial <-> first full
   t = cp_opaque(t);
@@ -426,6 +434,7 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     prev_pad = pad;
     prev_m = m;
   }
+  hook();
   if constexpr (PP::NF > 0) dif_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
 }
 
@@ -579,6 +588,21 @@ __device__ __forceinline__ void ld_root2(const uint64_t *__restrict__ tg, unsign
   wv = b.x;
   wvp = b.y;
 }
+// The root of node 1 (stage 0, the same for every slot) through the constant
+// address space: with a wave-uniform table pointer that is a scalar load,
+// which stays off the vector memory counter (LDS copies in flight are waited
+// on through that counter).  The tables do not change during a kernel.
+template <class TW>
+__device__ __forceinline__ void ld_root2_node1(TW tg, uint64_t &wu, uint64_t &wup, uint64_t &wv,
+                                               uint64_t &wvp) {
+  typedef __attribute__((address_space(4))) const cp_u64x2 const_u64x2;
+  const const_u64x2 *k = (const const_u64x2 *)(uintptr_t)(tg + 4);
+  const cp_u64x2 a = k[0], b = k[1];
+  wu = a.x;
+  wup = a.y;
+  wv = b.x;
+  wvp = b.y;
+}
 
 // One split-basis Gentleman-Sande butterfly: the group of A at global stage s
 // is pos >> (LOGN - s), its root at node 2^s + group.  At stage 0 the root is
@@ -586,9 +610,13 @@ __device__ __forceinline__ void ld_root2(const uint64_t *__restrict__ tg, unsign
 template <int LOGN, class TW>
 __device__ __forceinline__ void gsi_bfly(cp_gi &A, cp_gi &B, int s, int pos, TW __restrict__ tg,
                                          uint64_t p) {
-  const unsigned node = (1u << s) + ((unsigned)pos >> (LOGN - s));
+  // stage 0 has the single node 1 (pos < 2^LOGN): a uniform root, which the
+  // compiler cannot see when pos comes from an opaque thread index
   uint64_t wu, wup, wv, wvp;
-  ld_root2(tg, node, wu, wup, wv, wvp);
+  if (s == 0)
+    ld_root2_node1(tg, wu, wup, wv, wvp);
+  else
+    ld_root2(tg, (1u << s) + ((unsigned)pos >> (LOGN - s)), wu, wup, wv, wvp);
   const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
   A.re = cp_red2(A.re + B.re, p);
   A.imag = cp_red2(A.imag + B.imag, p);
@@ -657,9 +685,10 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
 // loaded the registers' layout with the map this sequence's first exchange
 // stores with, so the barrier before those stores can go as well.
 template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, bool AFTER_DIF = false, class Map,
-          class TW>
+          class TW, class Hook = NoHook>
 __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
-                                           TW __restrict__ tg, uint64_t p, const Map &map) {
+                                           TW __restrict__ tg, uint64_t p, const Map &map,
+                                           const Hook &hook = Hook()) {
   using PP = PassPlan<LSB, LSE>;
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
   t = cp_opaque(t);
@@ -678,6 +707,7 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
+    if (PP::R0 == 0 && i == 0) hook();
     gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0);
     prev_local = local;
     prev_pad = pad;
@@ -690,6 +720,7 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
       __syncthreads();
       lds_load<LOGM, LSB, PP::R0, MP.A, MP.S>(lds, x, t);
     }
+    hook();
     gsi_pass<LOGN, LOGM, LSB, PP::R0, SOFF>(x, t, tg, p, map);
   }
 }
@@ -705,6 +736,42 @@ constexpr int fused_min_waves() { return LOGN == 12 ? 4 : 1; }
 template <int LOGN>
 constexpr int rows_for() {
   return ((1 << LOGN) / EPT) >= 256 ? 1 : 256 / ((1 << LOGN) / EPT);
+}
+
+// ---- Operand copies into LDS by the DMA path --------------------------------
+// global_load_lds_dwordx4 writes a wave's 64 lanes x 16 bytes to LDS at
+// M0 + 16 * lane and holds no VGPRs while in flight, so a copy can be started
+// before a register pass and overlap it.  Each wave copies its threads' EPT
+// first-pass slots of one residue polynomial into its own area of EPT * 64
+// slots, lane-major (slot i of lane l at i * 64 + l), and only that wave reads
+// the area back, after lds_copy_wait (the wave's own memory counter), with no
+// workgroup barrier.  `area` must be wave-uniform.
+template <int LOGN>
+__device__ __forceinline__ void lds_copy_issue(cp_gi *area, const CP_GLOBAL cp_gi *src, int t) {
+  typedef __attribute__((address_space(3))) void lds_void;
+  t = cp_opaque(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+    __builtin_amdgcn_global_load_lds((const CP_GLOBAL void *)(src + first_index<LOGN, 0, LOGN>(t, i)),
+                                     (lds_void *)(area + i * 64), 16, 0, 0);
+}
+// The compiler does not order LDS reads after these copies by itself: the
+// wait is explicit, and the memory clobber keeps every later LDS read below it.
+__device__ __forceinline__ void lds_copy_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Makes the values of x count as computed at this point, so that the work
+// producing them (a register pass that loads in flight are meant to overlap)
+// is scheduled before the volatile statements that follow.
+__device__ __forceinline__ void pin_regs(cp_gi (&x)[EPT]) {
+  static_assert(EPT == 8, "one operand pair per slot");
+  asm volatile(""
+               : "+v"(x[0].re), "+v"(x[0].imag), "+v"(x[1].re), "+v"(x[1].imag),
+                 "+v"(x[2].re), "+v"(x[2].imag), "+v"(x[3].re), "+v"(x[3].imag),
+                 "+v"(x[4].re), "+v"(x[4].imag), "+v"(x[5].re), "+v"(x[5].imag),
+                 "+v"(x[6].re), "+v"(x[6].imag), "+v"(x[7].re), "+v"(x[7].imag));
+}
+__device__ __forceinline__ void lds_copy_read(cp_gi (&x)[EPT], const cp_gi *area, int lane) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = area[i * 64 + lane];
 }
 
 // ---- n = 2^15: one residue product of N = 16384 slots in one workgroup -----

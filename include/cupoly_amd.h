@@ -194,10 +194,16 @@ int cupoly_polymul_batch(cupoly_ctx *c, cupoly_gi *out, const cupoly_gi *a, cons
                          int batch);
 /* Launch choices of cupoly_polymul_batch on this context (no reference
  * counterpart): `rows` residue polynomials per workgroup for n = 2^11 .. 2^13
- * (1, 2, 4 or 8 with rows * n/2 <= 8192; 0 = built-in), and for n = 2^15
- * `persistent` = 1 (one workgroup per residue product, persistent) or -1
- * (two-kernel column/block split; 0 = persistent when the batch has at least
- * one residue product per CU). */
+ * (1, 2, 4 or 8 with rows * n/2 <= 8192; 0 = built-in), and `persistent`:
+ *  - n = 2^11 .. 2^14 with one residue polynomial per workgroup: 1 = the
+ *    persistent streaming form (a grid the GPU holds at once; operands copied
+ *    into LDS while the previous transform ends), -1 = one workgroup per
+ *    residue product; 0 = streaming at n = 2^14 when the batch has more
+ *    residue products than the GPU has CUs, one-shot otherwise;
+ *  - n = 2^15: 1 = one workgroup per residue product, persistent, -1 =
+ *    two-kernel column/block split; 0 = persistent when the batch has at
+ *    least one residue product per CU.
+ * Other ring degrees take persistent = 0 only (CUPOLY_ERR_ARG otherwise). */
 int cupoly_polymul_set_tuning(cupoly_ctx *c, int rows, int persistent);
 /* Autotuner for the batched product: times every available (rows,
  * persistent) choice for this ring degree over `reps` launches on the

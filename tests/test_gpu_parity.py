@@ -481,15 +481,17 @@ def test_sample_muladd_matches_oracle(gpu, k, n, batch):
 
 
 @pytest.mark.parametrize("n,k,batch,choices", [
-    (2048, 2, 40, [(1, 0), (2, 0), (4, 0), (8, 0)]),
-    (4096, 3, 20, [(1, 0), (2, 0), (4, 0)]),
-    (8192, 2, 12, [(1, 0), (2, 0)]),
+    (2048, 2, 40, [(1, -1), (2, 0), (4, 0), (8, 0), (1, 1)]),
+    (4096, 3, 20, [(1, -1), (2, 0), (4, 0), (1, 1)]),
+    (8192, 2, 12, [(1, -1), (2, 0), (1, 1)]),
+    (16384, 2, 10, [(0, 1), (0, -1)]),
     (32768, 3, 90, [(0, 1), (0, -1)]),
 ])
 def test_polymul_tuning_choices_and_autotune(gpu, n, k, batch, choices):
-    """Every launch choice of the batched product (rows per workgroup; for
-    n = 2^15 persistent or split) gives the same products, equal to the
-    oracle; the autotuner picks one of them and leaves the product in out."""
+    """Every launch choice of the batched product (rows per workgroup; the
+    persistent streaming form or one-shot workgroups; for n = 2^15 persistent
+    or split) gives the same products, equal to the oracle; the autotuner
+    picks one of them and leaves the product in out."""
     e, ctx = engine(k, k + 1, n)
     P = O.Params(k, k + 1, n, 65537)
     a = rand_res(P, P.q, (batch,), 500 + n)
@@ -513,6 +515,37 @@ def test_polymul_tuning_choices_and_autotune(gpu, n, k, batch, choices):
     cp.polymul_set_tuning(ctx, 0, 0)
     with pytest.raises(RuntimeError):
         cp.polymul_set_tuning(ctx, 16, 0)
-    if n != 32768:
-        with pytest.raises(RuntimeError):
-            cp.polymul_set_tuning(ctx, 0, 1)
+
+
+def test_polymul_persistent_choice_needs_n_2048_or_more(gpu):
+    e, ctx = engine(2, 3, 1024)
+    with pytest.raises(RuntimeError):
+        cp.polymul_set_tuning(ctx, 0, 1)
+
+
+@pytest.mark.parametrize("n,k,batch", [(16384, 4, 100), (4096, 3, 400)])
+def test_polymul_streaming_many_units_per_workgroup(gpu, n, k, batch):
+    """More residue products than the streaming grid holds, so each
+    workgroup runs several units, each with its operands copied into LDS
+    during the previous transform's last pass: equal to the one-shot form for
+    every product (not a multiple of the grid), and to the oracle for the first
+    and last."""
+    e, ctx = engine(k, k + 1, n)
+    P = O.Params(k, k + 1, n, 65537)
+    a = rand_res(P, P.q, (batch,), 900 + n)
+    b = rand_res(P, P.q, (batch,), 901 + n)
+    da, db = cp.to_device(a), cp.to_device(b)
+    outs = {}
+    for persistent in (-1, 1):
+        cp.polymul_set_tuning(ctx, 1 if n < 16384 else 0, persistent)
+        out = cp.empty(a.shape)
+        cp.polymul_batch(ctx, out, da, db)
+        outs[persistent] = cp.to_host(out)
+    assert np.array_equal(outs[1], outs[-1])
+    for i in (0, batch - 1):
+        assert np.array_equal(outs[1][i], O.polymul_batch(P, a[i:i + 1], b[i:i + 1])[0])
+    # in place: out is a
+    cp.polymul_set_tuning(ctx, 0, 1)
+    cp.polymul_batch(ctx, da, da, db)
+    assert np.array_equal(cp.to_host(da), outs[1])
+    cp.polymul_set_tuning(ctx, 0, 0)
