@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   // scheduling barriers keep the twist-table loads of one slot pair from being
   // hoisted above the previous pair's products (which would spill).
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = a[first_index<LOGN, 0, LOGN>(t, i)];
+  for (int i = 0; i < EPT; i++) xa[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)a + first_index<LOGN, 0, LOGN>(t, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xa[i], xa[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   int tb = t;
   asm volatile("" : "+v"(tb));
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xb[i] = b[first_index<LOGN, 0, LOGN>(tb, i)];
+  for (int i = 0; i < EPT; i++) xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tb, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(to, i);
       split_back(xa[i], h, hp, hj, hjp, p);
-      out[g] = xa[i];
+      st_gi_nt((CP_GLOBAL cp_gi *)out + g, xa[i]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -423,7 +423,7 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
     dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id, [&] {
       const int tq = cp_opaque(t);
 #pragma unroll
-      for (int i = 0; i < EPT; i++) xb[i] = ld_gi(ub + first_index<LOGN, 0, LOGN>(tq, i));
+      for (int i = 0; i < EPT; i++) xb[i] = ld_gi_nt(ub + first_index<LOGN, 0, LOGN>(tq, i));
     });
     pin_regs(xa);
     {
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       split_back(xa[i], h, hp, hj, hjp, p);
-      st_gi(uo + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
+      st_gi_nt(uo + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }

@@ -59,6 +59,18 @@ __device__ __forceinline__ cp_gi ld_gi(const CP_GLOBAL cp_gi *ptr) {
 __device__ __forceinline__ void st_gi(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
   *(CP_GLOBAL cp_u64x2 *)ptr = cp_u64x2{v.re, v.imag};
 }
+// Operands read once and results written once by a kernel (as opposed to
+// its tables and its own parked data, which are re-read): nontemporal
+// accesses (the nt cache policy), which keep them from displacing the tables
+// from the caches.
+__device__ __forceinline__ cp_gi ld_gi_nt(const CP_GLOBAL cp_gi *ptr) {
+  const cp_u64x2 v = __builtin_nontemporal_load((const CP_GLOBAL cp_u64x2 *)ptr);
+  return cp_gi{v.x, v.y};
+}
+__device__ __forceinline__ void st_gi_nt(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
+  __builtin_nontemporal_store(cp_u64x2{v.re, v.imag}, (CP_GLOBAL cp_u64x2 *)ptr);
+}
+#define CP_CPOL_NT 2  // cache-policy operand of the LDS-DMA builtin: nontemporal
 __device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw, unsigned e,
                                       uint64_t &w, uint64_t &wp) {
   const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)(tw + 2 * e);
@@ -753,7 +765,7 @@ __device__ __forceinline__ void lds_copy_issue(cp_gi *area, const CP_GLOBAL cp_g
 #pragma unroll
   for (int i = 0; i < EPT; i++)
     __builtin_amdgcn_global_load_lds((const CP_GLOBAL void *)(src + first_index<LOGN, 0, LOGN>(t, i)),
-                                     (lds_void *)(area + i * 64), 16, 0, 0);
+                                     (lds_void *)(area + i * 64), 16, 0, CP_CPOL_NT);
 }
 // The compiler does not order LDS reads after these copies by itself: the
 // wait is explicit, and the memory clobber keeps every later LDS read below it.
@@ -811,7 +823,7 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
-    cp_gi x = ld_gi(src + g), y = ld_gi(src + g + H);
+    cp_gi x = ld_gi_nt(src + g), y = ld_gi_nt(src + g + H);
     split_twist2(x, y, stw, g, g + H, J, Jp, p);
     uint64_t w, wp;
     ld_tw(twf, (unsigned)g, w, wp);
@@ -844,8 +856,8 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
               dd.imag);
     split_back(s, h, hp, hj, hjp, p);
     split_back(dd, h, hp, hj, hjp, p);
-    st_gi(out + g, s);
-    st_gi(out + g + H, dd);
+    st_gi_nt(out + g, s);
+    st_gi_nt(out + g + H, dd);
     __builtin_amdgcn_sched_barrier(0);
   }
 }

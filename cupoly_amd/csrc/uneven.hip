@@ -133,8 +133,8 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     const int g0 = first_index<LOGN, 0, LOGN>(ta, i), g1 = first_index<LOGN, 0, LOGN>(ta, i + 1);
-    xa[i] = ld_gi(ua + g0);
-    xa[i + 1] = ld_gi(ua + g1);
+    xa[i] = ld_gi_nt(ua + g0);
+    xa[i + 1] = ld_gi_nt(ua + g1);
     split_twist2(xa[i], xa[i + 1], stw, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -143,8 +143,8 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     const int g0 = first_index<LOGN, 0, LOGN>(tb, i), g1 = first_index<LOGN, 0, LOGN>(tb, i + 1);
-    xb[i] = ld_gi(ub + g0);
-    xb[i + 1] = ld_gi(ub + g1);
+    xb[i] = ld_gi_nt(ub + g0);
+    xb[i + 1] = ld_gi_nt(ub + g1);
     split_twist2(xb[i], xb[i + 1], stw, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -159,7 +159,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(to, i);
       split_back(xa[i], h, hp, hj, hjp, p);
-      st_gi(uc + g, xa[i]);
+      st_gi_nt(uc + g, xa[i]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
