@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   // scheduling barriers keep the twist-table loads of one slot pair from being
   // hoisted above the previous pair's products (which would spill).
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)a + first_index<LOGN, 0, LOGN>(t, i));
+  for (int i = 0; i < EPT; i++)
+    xa[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)a + first_index<LOGN, 0, LOGN>(t, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xa[i], xa[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
@@ -278,7 +279,8 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   int tb = t;
   asm volatile("" : "+v"(tb));
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tb, i));
+  for (int i = 0; i < EPT; i++)
+    xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tb, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),

@@ -47,11 +47,6 @@ __device__ __forceinline__ void ld_tw(const uint64_t *__restrict__ tw, unsigned 
   w = v.x;
   wp = v.y;
 }
-// Pointers typed in the global address space: code reached through an
-// out-of-line call (or pointers read from memory) would otherwise address
-// tables with flat instructions.
-#define CP_GLOBAL __attribute__((address_space(1)))
-typedef unsigned long long cp_u64x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ cp_gi ld_gi(const CP_GLOBAL cp_gi *ptr) {
   const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)ptr;
   return cp_gi{v.x, v.y};
@@ -59,18 +54,6 @@ __device__ __forceinline__ cp_gi ld_gi(const CP_GLOBAL cp_gi *ptr) {
 __device__ __forceinline__ void st_gi(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
   *(CP_GLOBAL cp_u64x2 *)ptr = cp_u64x2{v.re, v.imag};
 }
-// Operands read once and results written once by a kernel (as opposed to
-// its tables and its own parked data, which are re-read): nontemporal
-// accesses (the nt cache policy), which keep them from displacing the tables
-// from the caches.
-__device__ __forceinline__ cp_gi ld_gi_nt(const CP_GLOBAL cp_gi *ptr) {
-  const cp_u64x2 v = __builtin_nontemporal_load((const CP_GLOBAL cp_u64x2 *)ptr);
-  return cp_gi{v.x, v.y};
-}
-__device__ __forceinline__ void st_gi_nt(CP_GLOBAL cp_gi *ptr, const cp_gi &v) {
-  __builtin_nontemporal_store(cp_u64x2{v.re, v.imag}, (CP_GLOBAL cp_u64x2 *)ptr);
-}
-#define CP_CPOL_NT 2  // cache-policy operand of the LDS-DMA builtin: nontemporal
 __device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw, unsigned e,
                                       uint64_t &w, uint64_t &wp) {
   const cp_u64x2 v = *(const CP_GLOBAL cp_u64x2 *)(tw + 2 * e);

@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256)
   cp_acc_init(hi);
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const cp_gi v = y[(size_t)j * ng + c];
+    const cp_gi v = ld_gi_nt((const CP_GLOBAL cp_gi *)y + (size_t)j * ng + c);
     xr[j] = v.re;
     xi[j] = v.imag;
     uint64_t lr, li;
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256)
   double vr = 0.0, vi = 0.0;
   for (int i = 0; i < kl; i++) {
     const uint64_t b = H->b[i].p;
-    const cp_gi yv = yb[(size_t)i * ng + c];
+    const cp_gi yv = ld_gi_nt((const CP_GLOBAL cp_gi *)yb + (size_t)i * ng + c);
     uint64_t rr, ri;
     if (!slow) {
       scale_exact2(hr, hi, xr, xi, K, H->wint[i], yv.re, yv.imag, H->lam[i], b, rr, ri);
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256)
       r.re = submod_ref(sr[j], shoup2(v_re, H->Bqi[j], q), q);
       r.imag = submod_ref(si[j], shoup2(v_im, H->Bqi[j], q), q);
     }
-    o[(size_t)j * ng + c] = r;
+    st_gi_nt((CP_GLOBAL cp_gi *)o + (size_t)j * ng + c, r);
   }
 }
 
@@ -213,7 +213,8 @@ __device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restric
                                          uint64_t Jp, uint64_t p) {
   t = cp_opaque(t);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) x[i] = src[first_index<LOGN, 0, LOGN>(t, i)];
+  for (int i = 0; i < EPT; i++)
+    x[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)src + first_index<LOGN, 0, LOGN>(t, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(x[i], x[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
@@ -232,7 +233,7 @@ __device__ __forceinline__ void fti_store(cp_gi *__restrict__ dst, cp_gi (&x)[EP
     for (int i = 0; i < EPT; i++) {
       const int g = first_index<LOGN, 0, LOGN>(t, i);
       split_back(x[i], h, hp, hj, hjp, p);
-      dst[g] = x[i];
+      st_gi_nt((CP_GLOBAL cp_gi *)dst + g, x[i]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }

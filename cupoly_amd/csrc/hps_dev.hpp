@@ -72,7 +72,7 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
   for (int j = 0; j < CP_MAX_BASE; j++) {
     if (j < k) {
       const uint64_t q = H->q[j].p;
-      const cp_gi x = in[(size_t)j * qs + c];
+      const cp_gi x = ld_gi_nt((const CP_GLOBAL cp_gi *)in + (size_t)j * qs + c);
       ar[j] = shoup2(x.re, H->invQqi[j], q);
       ai[j] = shoup2(x.imag, H->invQqi[j], q);
     }
@@ -98,7 +98,7 @@ __device__ __forceinline__ void hps_q_to_b_slot(const cp_gi *in, size_t qs, cp_g
     cp_gi o;
     o.re = cp_acc_reduce(sr, b);
     o.imag = cp_acc_reduce(si, b);
-    out[(size_t)i * bs + c] = o;
+    st_gi_nt((CP_GLOBAL cp_gi *)out + (size_t)i * bs + c, o);
   }
 }
 

@@ -239,8 +239,9 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
       for (int r = 0; r < EPT; r++) {
         const int g = first_index<LOGN, 0, LOGN>(ts, r);
         split_back(acc[r], h, hp, hj, hjp, p);
-        const cp_gi v = ym[g];
-        om[g] = u64x2{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)};
+        const cp_gi v = ld_gi_nt((const CP_GLOBAL cp_gi *)ym + g);
+        __builtin_nontemporal_store(
+            u64x2{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)}, om + g);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
