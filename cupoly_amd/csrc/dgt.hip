@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
                     const cp_gi *__restrict__ b, int nunits, int nres,
                     const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
                     const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
-                    const uint64_t *__restrict__ sk_all) {
+                    const uint64_t *__restrict__ stwh_all, const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   // with one row per workgroup, keep the row (hence the residue and every
@@ -241,11 +241,13 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const cp_prime pr = primes[rid];
   const uint64_t p = pr.p;
   const uint64_t *twf = twf_all + (size_t)rid * N;
-  const uint64_t *rootp = root_all + (size_t)rid * N * 4, *stwp = stw_all + (size_t)rid * N * 4;
+  const uint64_t *rootp = root_all + (size_t)rid * N * 4, *stwp = stw_all + (size_t)rid * N * 4,
+                 *stwhp = stwh_all + (size_t)rid * N * 4;
   const uint64_t *sk = sk_all + (size_t)rid * 8;
   if (ROWS == 1) {
     rootp = uniform_ptr(rootp);
     stwp = uniform_ptr(stwp);
+    stwhp = uniform_ptr(stwhp);
     a = uniform_ptr(a + (size_t)unit * N);
     b = uniform_ptr(b + (size_t)unit * N);
     out = uniform_ptr(out + (size_t)unit * N);
@@ -256,6 +258,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
   const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)rootp;
   const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)stwp;
+  const CP_GLOBAL uint64_t *stwh = (const CP_GLOBAL uint64_t *)stwhp;
   const uint64_t J = sk[0], Jp = sk[1];
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tb, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
-    split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
+    split_twist2(xb[i], xb[i + 1], stwh, first_index<LOGN, 0, LOGN>(tb, i),
                  first_index<LOGN, 0, LOGN>(tb, i + 1), J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -293,13 +296,12 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
   int to = t;
   asm volatile("" : "+v"(to));
-  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
   if (active) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int g = first_index<LOGN, 0, LOGN>(to, i);
-      split_back(xa[i], h, hp, hj, hjp, p);
-      st_gi_nt((CP_GLOBAL cp_gi *)out + g, xa[i]);
+    for (int i = 0; i < EPT; i += 2) {
+      split_back_h2(xa[i], xa[i + 1], J, Jp, p);
+      st_gi_nt((CP_GLOBAL cp_gi *)out + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
+      st_gi_nt((CP_GLOBAL cp_gi *)out + first_index<LOGN, 0, LOGN>(to, i + 1), xa[i + 1]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -349,7 +351,8 @@ __global__ void __launch_bounds__(1024)
                   const cp_gi *__restrict__ b, int nunits, int nres,
                   const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
                   const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
-                  const uint64_t *__restrict__ sk_all, cp_gi *__restrict__ park_all) {
+                  const uint64_t *__restrict__ stwh_all, const uint64_t *__restrict__ sk_all,
+                  cp_gi *__restrict__ park_all) {
   constexpr int N = 16384;
   extern __shared__ cp_gi lds[];
   CP_GLOBAL cp_gi *park = (CP_GLOBAL cp_gi *)(park_all + (size_t)blockIdx.x * 3 * (N / 2));
@@ -362,6 +365,7 @@ __global__ void __launch_bounds__(1024)
                      (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N),
                      (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4),
                      (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4),
+                     (const CP_GLOBAL uint64_t *)(stwh_all + (size_t)rid * N * 4),
                      sk_all + (size_t)rid * 8);
     __syncthreads();
   }
@@ -383,7 +387,7 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
                      const cp_gi *__restrict__ b, int nunits, int nres,
                      const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
                      const uint64_t *__restrict__ root_all, const uint64_t *__restrict__ stw_all,
-                     const uint64_t *__restrict__ sk_all) {
+                     const uint64_t *__restrict__ stwh_all, const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   static_assert(T % 64 == 0 && PassPlan<0, LOGN>::NF > 0, "whole waves, a register last pass");
   static_assert(T * EPT <= lds_row<LOGN>(), "the copy areas fit the exchange array");
@@ -403,6 +407,8 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
         (const CP_GLOBAL uint64_t *)uniform_ptr(root_all + (size_t)rid * N * 4);
     const CP_GLOBAL uint64_t *stw =
         (const CP_GLOBAL uint64_t *)uniform_ptr(stw_all + (size_t)rid * N * 4);
+    const CP_GLOBAL uint64_t *stwh =
+        (const CP_GLOBAL uint64_t *)uniform_ptr(stwh_all + (size_t)rid * N * 4);
     const uint64_t *sk = sk_all + (size_t)rid * 8;
     const uint64_t J = sk[0], Jp = sk[1];
     const int next = unit + (int)gridDim.x;
@@ -432,7 +438,7 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
       const int tb = cp_opaque(t);
 #pragma unroll
       for (int i = 0; i < EPT; i += 2) {
-        split_twist2(xb[i], xb[i + 1], stw, first_index<LOGN, 0, LOGN>(tb, i),
+        split_twist2(xb[i], xb[i + 1], stwh, first_index<LOGN, 0, LOGN>(tb, i),
                      first_index<LOGN, 0, LOGN>(tb, i + 1), J, Jp, p);
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -446,13 +452,13 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
         lds_copy_issue<LOGN>(area, (const CP_GLOBAL cp_gi *)uniform_ptr(a + (size_t)next * N),
                              t);
     });
-    const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
     CP_GLOBAL cp_gi *uo = (CP_GLOBAL cp_gi *)uniform_ptr(out + (size_t)unit * N);
     const int to = cp_opaque(t);
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      split_back(xa[i], h, hp, hj, hjp, p);
+    for (int i = 0; i < EPT; i += 2) {
+      split_back_h2(xa[i], xa[i + 1], J, Jp, p);
       st_gi_nt(uo + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
+      st_gi_nt(uo + first_index<LOGN, 0, LOGN>(to, i + 1), xa[i + 1]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -605,7 +611,7 @@ static void launch_fused(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi 
   const size_t lds = (size_t)R * lds_row<LOGN>() * sizeof(cp_gi);
   hipLaunchKernelGGL((k_polymul_fused<LOGN, R>), dim3((nunits + R - 1) / R), dim3(T * R), lds,
                      c->stream, out, a, b, nunits, e->k, e->d_primes, e->d_tw_fwd,
-                     e->d_split_root, e->d_split_tw, e->d_split_k);
+                     e->d_split_root, e->d_split_tw, e->d_split_tw_h, e->d_split_k);
 }
 
 // Persistent (streaming) form: as many workgroups as the GPU holds at once.
@@ -624,7 +630,7 @@ static int launch_stream(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi 
   const int grid = nunits < slots ? nunits : (int)slots;
   hipLaunchKernelGGL(k_polymul_stream<LOGN>, dim3(grid), dim3(T), lds, c->stream, out, a, b,
                      nunits, e->k, e->d_primes, e->d_tw_fwd, e->d_split_root, e->d_split_tw,
-                     e->d_split_k);
+                     e->d_split_tw_h, e->d_split_k);
   return CUPOLY_OK;
 }
 
@@ -687,7 +693,7 @@ static int polymul_t(cupoly_ctx *c, cp_gi *out, const cp_gi *a, const cp_gi *b, 
     const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
     hipLaunchKernelGGL(k_polymul_16k, dim3(grid), dim3(1024), lds, c->stream, out, a, b, nunits,
                        nres, e->d_primes, e->d_tw_fwd, e->d_split_root, e->d_split_tw,
-                       e->d_split_k, (cp_gi *)ws);
+                       e->d_split_tw_h, e->d_split_k, (cp_gi *)ws);
     CP_LAUNCH_CHECK();
   } else {
     // out <- a, scratch <- b (caller inputs are not modified); transforms in place

@@ -562,6 +562,17 @@ __device__ __forceinline__ void split_back(cp_gi &x, uint64_t h, uint64_t hp, ui
                                            uint64_t hjp, uint64_t p) {
   cp_shoup2(x.re + x.imag, h, hp, x.re + p - x.imag, hj, hjp, p, x.re, x.imag);
 }
+// The same for a product whose second operand was twisted with the table
+// that carries h (d_split_tw_h): (U, V) already hold the factor h, so
+// c_g = U + V and c_{g+N} = (U - V) J^-1 = (V - U) J (J^2 = -1).  Two slots,
+// so that their products by J pair up.
+__device__ __forceinline__ void split_back_h2(cp_gi &x, cp_gi &y, uint64_t J, uint64_t Jp,
+                                              uint64_t p) {
+  const uint64_t xr = x.re, yr = y.re;
+  x.re = cp_red2(xr + x.imag, p);
+  y.re = cp_red2(yr + y.imag, p);
+  cp_shoup2(x.imag + p - xr, J, Jp, y.imag + p - yr, J, Jp, p, x.imag, y.imag);
+}
 
 __device__ __forceinline__ void ld_root2(const CP_GLOBAL uint64_t *__restrict__ tg, unsigned node,
                                          uint64_t &wu, uint64_t &wup, uint64_t &wv,
@@ -818,8 +829,9 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
 }
 
 // split-tree inverse stage 0 (root r_0^-1 of node 1, the same for every
-// slot) of slots g and g + 8192 and the conversion back to coefficients,
-// written to out; sk = the residue's split constants
+// slot) of slots g and g + 8192 and the conversion back to coefficients
+// (the product's second operand carried h: split_back_h2), written to out;
+// sk = the residue's split constants
 __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&x)[EPT],
                                             const cp_gi (&y)[EPT], int t,
                                             const CP_GLOBAL uint64_t *root,
@@ -827,8 +839,8 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
   constexpr int H = 8192;
   t = cp_opaque(t);
   uint64_t wu, wup, wv, wvp;
-  ld_root2(root, 1u, wu, wup, wv, wvp);
-  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
+  ld_root2_node1(root, wu, wup, wv, wvp);
+  const uint64_t J = sk[0], Jp = sk[1];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
@@ -837,8 +849,7 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
     s.imag = cp_red2(x[i].imag + y[i].imag, p);
     cp_shoup2(x[i].re + p - y[i].re, wu, wup, x[i].imag + p - y[i].imag, wv, wvp, p, dd.re,
               dd.imag);
-    split_back(s, h, hp, hj, hjp, p);
-    split_back(dd, h, hp, hj, hjp, p);
+    split_back_h2(s, dd, J, Jp, p);
     st_gi_nt(out + g, s);
     st_gi_nt(out + g + H, dd);
     __builtin_amdgcn_sched_barrier(0);
@@ -853,6 +864,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
                                                  const CP_GLOBAL uint64_t *twf,
                                                  const CP_GLOBAL uint64_t *root,
                                                  const CP_GLOBAL uint64_t *stw,
+                                                 const CP_GLOBAL uint64_t *stwh,
                                                  const uint64_t *sk) {
   constexpr int H = 8192;
   const int t = (int)threadIdx.x;
@@ -870,7 +882,7 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   }
   // b: per half, spectrum, product with a's (P0 / P1), inverse stages 13..1;
   // half 0's result waits in P0
-  stage0_16k(x, y, b, t, twf, stw, J, Jp, p);
+  stage0_16k(x, y, b, t, twf, stwh, J, Jp, p);
   park_st(park, 2, t, y);
 #pragma nounroll
   for (int h = 0; h < 2; h++) {

@@ -256,8 +256,8 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
   // -1; xi+- = xi.re +- J xi.im; the inverse roots r^-1 of the split tree of
   // X^ng -+ J come from its leaves xi+-^-1 omega^-bitrev(l) by squaring:
   // r^-1 at (s, g) is the leaf-side value at (s + 1, 2g).
-  std::vector<uint64_t> spt((size_t)nres * ng * 4), spr((size_t)nres * ng * 4),
-      spk((size_t)nres * 8, 0);
+  std::vector<uint64_t> spt((size_t)nres * ng * 4), spth((size_t)nres * ng * 4),
+      spr((size_t)nres * ng * 4), spk((size_t)nres * 8, 0);
   {
     int lg = 0;
     while ((1 << lg) < ng) lg++;
@@ -274,10 +274,14 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
         delete e;
         return cp_fail(CUPOLY_ERR_UNSUPPORTED, "split basis: root tables inconsistent for this prime");
       }
+      // the second operand of a product is twisted by h xi+-^g (h = 1/2ng): the
+      // product's spectrum then carries the inverse's 1/2ng already
+      const uint64_t h = invm((uint64_t)(2 * ng) % p, p);
       uint64_t f[2] = {1, 1};
       for (int g = 0; g < ng; g++) {
         for (int c = 0; c < 2; c++) {
           shoup_pair(f[c], p, &spt[((size_t)r * ng + g) * 4 + 2 * c]);
+          shoup_pair(mulm(f[c], h, p), p, &spth[((size_t)r * ng + g) * 4 + 2 * c]);
           f[c] = mulm(f[c], xs[c], p);
         }
       }
@@ -299,7 +303,6 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
           std::swap(cur, nxt);
         }
       }
-      const uint64_t h = invm((uint64_t)(2 * ng) % p, p);
       shoup_pair(J, p, &spk[(size_t)r * 8]);
       shoup_pair(h, p, &spk[(size_t)r * 8 + 2]);
       shoup_pair(mulm(h, invm(J, p), p), p, &spk[(size_t)r * 8 + 4]);
@@ -317,6 +320,7 @@ extern "C" int cupoly_engine_create(int k, int kl, int n, uint64_t t, cupoly_eng
       (rc = up((void **)&e->d_twist_fwd, tsf.data(), tsf.size() * 8)) ||
       (rc = up((void **)&e->d_twist_inv, tsi.data(), tsi.size() * 8)) ||
       (rc = up((void **)&e->d_split_tw, spt.data(), spt.size() * 8)) ||
+      (rc = up((void **)&e->d_split_tw_h, spth.data(), spth.size() * 8)) ||
       (rc = up((void **)&e->d_split_root, spr.data(), spr.size() * 8)) ||
       (rc = up((void **)&e->d_split_k, spk.data(), spk.size() * 8)) ||
       (rc = up((void **)&e->d_hps, &e->h_hps, sizeof(cp_hps))) ||
@@ -336,7 +340,7 @@ extern "C" int cupoly_engine_destroy(cupoly_engine *e) {
   (void)hipDeviceSynchronize();
   void *ptrs[] = {e->d_primes,    e->d_tw_fwd,   e->d_tw_inv,       e->d_twist_fwd,
                   e->d_twist_inv, e->d_split_tw, e->d_split_root, e->d_split_k,
-                  e->d_hps,       e->d_crt};
+                  e->d_hps,       e->d_crt,      e->d_split_tw_h};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete e;

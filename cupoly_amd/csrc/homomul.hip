@@ -222,18 +222,20 @@ __device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restric
     __builtin_amdgcn_sched_barrier(0);
   }
 }
+// (the d operands were twisted with the table that carries 1/2N, so every
+// product row holds it: split_back_h2)
 template <int LOGN>
 __device__ __forceinline__ void fti_store(cp_gi *__restrict__ dst, cp_gi (&x)[EPT], int t,
                                           const uint64_t *__restrict__ sk, uint64_t p,
                                           bool active) {
   t = cp_opaque(t);
-  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
+  const uint64_t J = sk[0], Jp = sk[1];
   if (active) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int g = first_index<LOGN, 0, LOGN>(t, i);
-      split_back(x[i], h, hp, hj, hjp, p);
-      st_gi_nt((CP_GLOBAL cp_gi *)dst + g, x[i]);
+    for (int i = 0; i < EPT; i += 2) {
+      split_back_h2(x[i], x[i + 1], J, Jp, p);
+      st_gi_nt((CP_GLOBAL cp_gi *)dst + first_index<LOGN, 0, LOGN>(t, i), x[i]);
+      st_gi_nt((CP_GLOBAL cp_gi *)dst + first_index<LOGN, 0, LOGN>(t, i + 1), x[i + 1]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -244,7 +246,8 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     k_hom_fti(cp_gi *__restrict__ Y, const cp_gi *__restrict__ XB, int batch, int nres, int k,
               cp_cts ct1, cp_cts ct2, const cp_prime *__restrict__ primes,
               const uint64_t *__restrict__ twf_all, const uint64_t *__restrict__ root_all,
-              const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ sk_all) {
+              const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ stwh_all,
+              const uint64_t *__restrict__ sk_all) {
   constexpr int N = 1 << LOGN, T = N / EPT;
   extern __shared__ cp_gi lds_all[];
   const int row = ROWS == 1 ? 0 : (int)(threadIdx.x / T);
@@ -262,6 +265,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   const uint64_t *twf = twf_all + (size_t)rid * N;
   const CP_GLOBAL uint64_t *root = (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4);
   const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const CP_GLOBAL uint64_t *stwh = (const CP_GLOBAL uint64_t *)(stwh_all + (size_t)rid * N * 4);
   const uint64_t *sk = sk_all + (size_t)rid * 8;
   const uint64_t J = sk[0], Jp = sk[1];
   // a Q residue comes straight from its ciphertext, a B residue from the
@@ -294,7 +298,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   cp_gi xa[EPT], xb[EPT];
   fti_load<LOGN>(xa, c0, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c0, held
-  fti_load<LOGN>(xb, d0, t, stw, J, Jp, p);
+  fti_load<LOGN>(xb, d0, t, stwh, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d0
   if (active) {
 #pragma unroll
@@ -304,7 +308,7 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   for (int i = 0; i < EPT; i++) xb[i] = split_mul(xa[i], xb[i], pr);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xb, lds, t, root, p, id);  // right after S_d0's passes
   fti_store<LOGN>(y0, xb, t, sk, p, active);
-  fti_load<LOGN>(xb, d1, t, stw, J, Jp, p);
+  fti_load<LOGN>(xb, d1, t, stwh, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d1, held
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -471,7 +475,8 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(1024)
     k_dgt_fwd_tm16(cp_gi *__restrict__ X, int nunits, int nres, int k, cp_cts ct1, cp_cts ct2,
                    const cp_prime *__restrict__ primes, const uint64_t *__restrict__ twf_all,
-                   const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ sk_all) {
+                   const uint64_t *__restrict__ stw_all, const uint64_t *__restrict__ stwh_all,
+                   const uint64_t *__restrict__ sk_all) {
   constexpr int N = 16384, H = 8192;
   extern __shared__ cp_gi lds[];
   const int t = (int)threadIdx.x;
@@ -481,12 +486,14 @@ __global__ void __launch_bounds__(1024)
   const int unit = (u - rid * per_r) * nres + rid;
   const uint64_t p = primes[rid].p;
   const CP_GLOBAL uint64_t *twf = (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N);
-  const CP_GLOBAL uint64_t *stw = (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4);
+  const int pw = unit / nres, pair = pw >> 2, which = pw & 3;  // 0 c0, 1 c1, 2 d0, 3 d1
+  // the d operands take the twists that carry 1/2N (split_back_h2 at the end)
+  const CP_GLOBAL uint64_t *stw =
+      (const CP_GLOBAL uint64_t *)((which >= 2 ? stwh_all : stw_all) + (size_t)rid * N * 4);
   const uint64_t J = sk_all[(size_t)rid * 8], Jp = sk_all[(size_t)rid * 8 + 1];
   CP_GLOBAL cp_gi *poly = (CP_GLOBAL cp_gi *)(X + (size_t)unit * N);
   const CP_GLOBAL cp_gi *src = poly;
   if (rid < k) {  // a Q residue comes straight from its ciphertext
-    const int pw = unit / nres, pair = pw >> 2, which = pw & 3;
     src = (const CP_GLOBAL cp_gi *)((which < 2 ? ct1.at(pair) : ct2.at(pair)) +
                                      (size_t)((which & 1) * k + rid) * N);
   }
@@ -552,7 +559,7 @@ static int launch_fti(cupoly_ctx *c, cp_gi *Y, const cp_gi *XB, int batch, cp_ct
   const int nunits = batch * e->nres;
   hipLaunchKernelGGL((k_hom_fti<LOGN, R>), dim3((nunits + R - 1) / R), dim3(THREADS), lds,
                      c->stream, Y, XB, batch, e->nres, e->k, ct1, ct2, e->d_primes, e->d_tw_fwd,
-                     e->d_split_root, e->d_split_tw, e->d_split_k);
+                     e->d_split_root, e->d_split_tw, e->d_split_tw_h, e->d_split_k);
   CP_LAUNCH_CHECK();
   return CUPOLY_OK;
 }
@@ -800,7 +807,7 @@ static int hom_mul_run(cupoly_ctx *c, cp_cts out, cp_cts ct1, cp_cts ct2, int ba
     const size_t lds = lds_row<13>() * sizeof(cp_gi);
     hipLaunchKernelGGL(k_dgt_fwd_tm16, dim3(batch * 4 * nres), dim3(1024), lds, c->stream, X,
                        batch * 4 * nres, nres, k, ct1, ct2, e->d_primes, e->d_tw_fwd,
-                       e->d_split_tw, e->d_split_k);
+                       e->d_split_tw, e->d_split_tw_h, e->d_split_k);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tensor_idgt16, dim3(batch * 3 * nres), dim3(1024), lds, c->stream, Y, X,
                        batch * 3 * nres, nres, e->d_primes, e->d_split_root, e->d_split_k);

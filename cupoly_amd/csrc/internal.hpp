@@ -76,6 +76,7 @@ struct cupoly_engine {
   uint64_t *d_twist_inv = nullptr;   // [nres][ng][6]     xi^-n / ng
   // split-basis product tables (dgt_dev.hpp), per residue r:
   uint64_t *d_split_tw = nullptr;    // [nres][ng][4]     xi+^g, xi-^g (Shoup pairs)
+  uint64_t *d_split_tw_h = nullptr;  // [nres][ng][4]     the same times 1/(2 ng)
   uint64_t *d_split_root = nullptr;  // [nres][ng][4]     r_u^-1, r_v^-1 at node 2^s + group
   uint64_t *d_split_k = nullptr;     // [nres][8]         J, 1/2N, 1/2JN (Shoup pairs)
   cp_hps h_hps;                      // host copy

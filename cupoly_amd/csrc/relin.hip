@@ -87,7 +87,11 @@ __global__ void __launch_bounds__(256)
   const size_t at = (size_t)blockIdx.y * ng + c;
   const cp_gi x = d[at];
   const uint64_t t = cp_shoup(x.imag, sk[0], sk[1], p);
-  ds[at] = cp_gi{cp_add(x.re, t, p), cp_sub(x.re, t, p)};
+  // times h = 1/2N (sk[2], sk[3]): every product with the key then carries the
+  // inverse transform's scale, and the conversion back is split_back_h2
+  cp_gi v;
+  cp_shoup2(cp_add(x.re, t, p), sk[2], sk[3], cp_sub(x.re, t, p), sk[2], sk[3], p, v.re, v.imag);
+  ds[at] = v;
 }
 
 // Digit i of coefficient-domain word x (residue i of y2), in residue j.
@@ -231,14 +235,13 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
     }
     gsi_passes<LOGN, LOGN, 0, LOGN, 0>(acc, lds, t, root, p, id);
     const int ts = cp_opaque(t);
-    const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
     if (active) {
       const cp_gi *ym = y + ((size_t)m * k + j) * N;
       global_gi *om = o + ((size_t)m * k + j) * N;
 #pragma unroll
       for (int r = 0; r < EPT; r++) {
+        if (r % 2 == 0) split_back_h2(acc[r], acc[r + 1], J, Jp, p);
         const int g = first_index<LOGN, 0, LOGN>(ts, r);
-        split_back(acc[r], h, hp, hj, hjp, p);
         const cp_gi v = ld_gi_nt((const CP_GLOBAL cp_gi *)ym + g);
         __builtin_nontemporal_store(
             u64x2{cp_add(v.re, acc[r].re, p), cp_add(v.imag, acc[r].imag, p)}, om + g);

@@ -44,11 +44,13 @@ struct cp_tile {
 };
 
 // Per degree class: forward twiddles, and the split-basis tables (inverse
-// roots, twists, constants; dgt_dev.hpp).
+// roots, twists, the second operand's twists with 1/2N, constants;
+// dgt_dev.hpp).
 struct cp_class_tables {
   const uint64_t *twf[UNEVEN_NCLASS];
   const uint64_t *root[UNEVEN_NCLASS];
   const uint64_t *stw[UNEVEN_NCLASS];
+  const uint64_t *stwh[UNEVEN_NCLASS];
   const uint64_t *sk[UNEVEN_NCLASS];
   const cp_prime *primes[UNEVEN_NCLASS];
 };
@@ -110,8 +112,8 @@ __device__ __forceinline__ cp_prime ld_prime(const cp_prime *primes, int i) {
 template <int LOGN>
 __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, int nrows,
                                           const uint64_t *twf_c, const uint64_t *root_c,
-                                          const uint64_t *stw_c, const uint64_t *sk_c,
-                                          const cp_prime *primes) {
+                                          const uint64_t *stw_c, const uint64_t *stwh_c,
+                                          const uint64_t *sk_c, const cp_prime *primes) {
   constexpr int N = 1 << LOGN, TPR = N / EPT, R = UNEVEN_SLOTS / N;
   static_assert(R * TPR == UNEVEN_THREADS, "tile shape");
   const int row = (int)threadIdx.x / TPR, t = (int)threadIdx.x % TPR;
@@ -123,6 +125,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   const CP_GLOBAL uint64_t *twf = glb(twf_c) + (size_t)u.rid * N;
   const CP_GLOBAL uint64_t *root = glb(root_c) + (size_t)u.rid * N * 4;
   const CP_GLOBAL uint64_t *stw = glb(stw_c) + (size_t)u.rid * N * 4;
+  const CP_GLOBAL uint64_t *stwh = glb(stwh_c) + (size_t)u.rid * N * 4;
   const CP_GLOBAL uint64_t *sk = glb(sk_c) + (size_t)u.rid * 8;
   const uint64_t J = sk[0], Jp = sk[1];
   const CP_GLOBAL cp_gi *ua = glb(u.a), *ub = glb(u.b);
@@ -145,7 +148,7 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
     const int g0 = first_index<LOGN, 0, LOGN>(tb, i), g1 = first_index<LOGN, 0, LOGN>(tb, i + 1);
     xb[i] = ld_gi_nt(ub + g0);
     xb[i + 1] = ld_gi_nt(ub + g1);
-    split_twist2(xb[i], xb[i + 1], stw, g0, g1, J, Jp, p);
+    split_twist2(xb[i], xb[i + 1], stwh, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
@@ -153,13 +156,12 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
   for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
   const int to = cp_opaque(t);
-  const uint64_t h = sk[2], hp = sk[3], hj = sk[4], hjp = sk[5];
   if (active) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int g = first_index<LOGN, 0, LOGN>(to, i);
-      split_back(xa[i], h, hp, hj, hjp, p);
-      st_gi_nt(uc + g, xa[i]);
+    for (int i = 0; i < EPT; i += 2) {
+      split_back_h2(xa[i], xa[i + 1], J, Jp, p);
+      st_gi_nt(uc + first_index<LOGN, 0, LOGN>(to, i), xa[i]);
+      st_gi_nt(uc + first_index<LOGN, 0, LOGN>(to, i + 1), xa[i + 1]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -169,14 +171,15 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
 // the 3 * 8192 parked slots.
 __device__ __forceinline__ void uneven_16k(const cp_unit *units, int unit0, const uint64_t *twf_c,
                                         const uint64_t *root_c, const uint64_t *stw_c,
-                                        const uint64_t *sk_c, const cp_prime *primes,
-                                        cp_gi *scratch) {
+                                        const uint64_t *stwh_c, const uint64_t *sk_c,
+                                        const cp_prime *primes, cp_gi *scratch) {
   constexpr int N = 16384;
   const cp_unit u = ld_unit(units, unit0);
   const cp_prime pr = ld_prime(primes, u.rid);
   polymul_16k_unit(glb(u.c), glb(u.a), glb(u.b), glb(scratch), uneven_lds, pr,
                    glb(twf_c) + (size_t)u.rid * N, glb(root_c) + (size_t)u.rid * N * 4,
-                   glb(stw_c) + (size_t)u.rid * N * 4, sk_c + (size_t)u.rid * 8);
+                   glb(stw_c) + (size_t)u.rid * N * 4, glb(stwh_c) + (size_t)u.rid * N * 4,
+                   sk_c + (size_t)u.rid * 8);
 }
 
 __global__ void __launch_bounds__(UNEVEN_THREADS)
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(UNEVEN_THREADS)
     if (ti >= ntiles) break;  // uniform: every wave of the workgroup leaves
     const cp_tile tl = tiles[ti];
     switch (tl.cls) {
-#define CP_UNEVEN_TABS(c) T.twf[c], T.root[c], T.stw[c], T.sk[c], T.primes[c]
+#define CP_UNEVEN_TABS(c) T.twf[c], T.root[c], T.stw[c], T.stwh[c], T.sk[c], T.primes[c]
       case 0: uneven_small<10>(units, tl.unit0, tl.nrows, CP_UNEVEN_TABS(0)); break;
       case 1: uneven_small<11>(units, tl.unit0, tl.nrows, CP_UNEVEN_TABS(1)); break;
       case 2: uneven_small<12>(units, tl.unit0, tl.nrows, CP_UNEVEN_TABS(2)); break;
@@ -266,6 +269,7 @@ extern "C" int cupoly_uneven_create(cupoly_ctx *c, cupoly_engine *const *engines
     u->tabs.twf[cl] = e->d_tw_fwd;
     u->tabs.root[cl] = e->d_split_root;
     u->tabs.stw[cl] = e->d_split_tw;
+    u->tabs.stwh[cl] = e->d_split_tw_h;
     u->tabs.sk[cl] = e->d_split_k;
     u->tabs.primes[cl] = e->d_primes;
     int rc = cupoly_ctx_create(e, (void *)c->stream, &u->class_ctx[cl]);
