@@ -415,7 +415,10 @@ __device__ __forceinline__ void dif_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     prev_m = MP;
   }
   bool prev_local = false, prev_pad = false;  // previous exchange of this loop
-#pragma nounroll
+  // unrolled: each pass's stage, twiddle indices, LDS map and barrier kind
+  // are then compile-time (immediate LDS offsets, no index arithmetic per
+  // slot): 4.6% faster products than the rolled loop, at 1.6x the code
+#pragma unroll
   for (int i = 0; i < PP::NF - 1; i++) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
@@ -445,7 +448,7 @@ __device__ __forceinline__ void dit_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   if constexpr (PP::NF > 0) dit_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tw, p, map);
   bool prev_local = false, prev_pad = false;
   LdsMap prev_m{-1, -1};  // map of the previous exchange of this sequence (none yet)
-#pragma nounroll
+#pragma unroll  // as in dif_passes
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
@@ -704,7 +707,7 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
   LdsMap prev_m = AFTER_DIF && PP::NF >= 2
                       ? lds_map_for(LOGM - (PP::LSF + LOG_EPT * (PP::NF - 2)) - 2 * LOG_EPT)
                       : LdsMap{-1, -1};
-#pragma nounroll
+#pragma unroll  // as in dif_passes
   for (int i = PP::NF - 2; i >= 0; i--) {
     const int ls0 = PP::LSF + LOG_EPT * i;
     const LdsMap m = lds_map_for(LOGM - ls0 - 2 * LOG_EPT);
