@@ -6,8 +6,13 @@
 #include "internal.hpp"
 
 #define EPT 8       // Gaussian slots per thread
+// A scheduling barrier after every radix-2 stage of a pass keeps one stage's
+// twiddle loads in flight at a time.  Since the passes are unrolled, the
+// compiler's own schedule across stages is faster for the products and the
+// seven-transform hom-mul kernel (1.0% and 2.0%); the relinearization kernel,
+// which holds more values live, keeps the barriers (relin.hip sets 1).
 #ifndef CP_STAGE_BARRIER
-#define CP_STAGE_BARRIER 1
+#define CP_STAGE_BARRIER 0
 #endif
 #define LOG_EPT 3
 
@@ -669,11 +674,23 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
   for (int h = LOG_EPT - 1; h >= 0; h--) {
     const int half = EPT >> (h + 1);
     const int s = ls0 + h + SOFF;
+    // block maps (map(l) = map(0) + l, map(0) a multiple of the local size,
+    // LOGN = LOGM + SOFF): the group of local slot base + (q << lst) at this
+    // stage is (map(0) >> (lst + 3 - h)) + ((t >> lst) << h) + (q >> (3 - h)),
+    // since base has no bits in [lst, lst + 3): one node per stage plus a
+    // compile-time offset per butterfly
+    const unsigned node0 =
+        (1u << s) + ((unsigned)map(0) >> (LOGN - s)) + (((unsigned)t >> lst) << h);
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       if (q & half) continue;
-      const int pos = map(base + (q << lst));
-      const unsigned node = (1u << s) + ((unsigned)pos >> (LOGN - s));
+      unsigned node;
+      if constexpr (Map::kLocalMask && LOGN == LOGM + SOFF) {
+        node = node0 + (unsigned)(q >> (LOG_EPT - h));
+      } else {
+        const int pos = map(base + (q << lst));
+        node = (1u << s) + ((unsigned)pos >> (LOGN - s));
+      }
       uint64_t wu, wup, wv, wvp;
       ld_root2(tg, node, wu, wup, wv, wvp);
       cp_gi &A = x[q];

@@ -30,6 +30,9 @@
 //     streaming dot-product kernel, batched inverse transforms, one add.
 #include <string>
 
+// one radix-2 stage's twiddle loads in flight at a time (dgt_dev.hpp): 1.4%
+// faster here than the compiler's schedule across stages
+#define CP_STAGE_BARRIER 1
 #include "dgt_dev.hpp"
 
 struct cupoly_relin_key {
