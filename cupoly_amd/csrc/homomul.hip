@@ -208,13 +208,17 @@ __global__ void __launch_bounds__(256)
 // dgt_dev.hpp: the spectra never leave the workgroup except through its own
 // parks, so the basis does not show in Y.
 template <int LOGN>
-__device__ __forceinline__ void fti_load(cp_gi (&x)[EPT], const cp_gi *__restrict__ src, int t,
-                                         const CP_GLOBAL uint64_t *__restrict__ stw, uint64_t J,
-                                         uint64_t Jp, uint64_t p) {
+__device__ __forceinline__ void fti_raw(cp_gi (&x)[EPT], const cp_gi *__restrict__ src, int t) {
   t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i++)
     x[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)src + first_index<LOGN, 0, LOGN>(t, i));
+}
+template <int LOGN>
+__device__ __forceinline__ void fti_twist(cp_gi (&x)[EPT], int t,
+                                          const CP_GLOBAL uint64_t *__restrict__ stw, uint64_t J,
+                                          uint64_t Jp, uint64_t p) {
+  t = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(x[i], x[i + 1], stw, first_index<LOGN, 0, LOGN>(t, i),
@@ -296,9 +300,18 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
   MapIdentity id;
   cp_gi xa[EPT], xb[EPT];
-  fti_load<LOGN>(xa, c0, t, stw, J, Jp, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c0, held
-  fti_load<LOGN>(xb, d0, t, stwh, J, Jp, p);
+  // Operand loads overlap register passes: d0 goes into registers during
+  // S_c0's last pass; d1 and c1 are copied into LDS (lds_copy_issue, each
+  // wave into its own area) during the last pass of the transform before
+  // theirs, once a barrier has freed the array.
+  static_assert(T % 64 == 0, "whole waves per row");
+  cp_gi *area = lds + __builtin_amdgcn_readfirstlane(t >> 6) * (EPT * 64);
+  const int lane = t & 63;
+  fti_raw<LOGN>(xa, c0, t);
+  fti_twist<LOGN>(xa, t, stw, J, Jp, p);
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id,  // S_c0, held
+                                     [&] { fti_raw<LOGN>(xb, d0, t); });
+  fti_twist<LOGN>(xb, t, stwh, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d0
   if (active) {
 #pragma unroll
@@ -306,17 +319,29 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
   }
 #pragma unroll
   for (int i = 0; i < EPT; i++) xb[i] = split_mul(xa[i], xb[i], pr);
-  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xb, lds, t, root, p, id);  // right after S_d0's passes
+  gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xb, lds, t, root, p, id, [&] {  // right after S_d0's passes
+    __syncthreads();
+    lds_copy_issue<LOGN>(area, (const CP_GLOBAL cp_gi *)d1, t);
+  });
   fti_store<LOGN>(y0, xb, t, sk, p, active);
-  fti_load<LOGN>(xb, d1, t, stwh, J, Jp, p);
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);  // S_d1, held
+  lds_copy_wait();
+  lds_copy_read(xb, area, lane);
+  fti_twist<LOGN>(xb, t, stwh, J, Jp, p);
+  // S_d1, held; the first exchange's barrier orders every wave's read of its
+  // copy before the array is overwritten
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id, [&] {
+    __syncthreads();
+    lds_copy_issue<LOGN>(area, (const CP_GLOBAL cp_gi *)c1, t);
+  });
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const cp_gi w = split_mul(xa[i], xb[i], pr);
     if (active) y2[i * T + cp_opaque(t)] = w;
     __builtin_amdgcn_sched_barrier(0);
   }
-  fti_load<LOGN>(xa, c1, t, stw, J, Jp, p);
+  lds_copy_wait();
+  lds_copy_read(xa, area, lane);
+  fti_twist<LOGN>(xa, t, stw, J, Jp, p);
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);  // S_c1
   const int tp = cp_opaque(t);
 #pragma unroll
