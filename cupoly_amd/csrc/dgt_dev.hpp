@@ -602,15 +602,17 @@ __device__ __forceinline__ void ld_root2(const uint64_t *__restrict__ tg, unsign
   wv = b.x;
   wvp = b.y;
 }
-// The root of node 1 (stage 0, the same for every slot) through the constant
-// address space: with a wave-uniform table pointer that is a scalar load,
-// which stays off the vector memory counter (LDS copies in flight are waited
-// on through that counter).  The tables do not change during a kernel.
+// The root of a wave-uniform node (node 1 at stage 0, the same for every
+// slot; any node whose group does not depend on the slot) through the
+// constant address space: with a wave-uniform table pointer that is a scalar
+// load, which stays off the vector memory counter (loads and LDS copies in
+// flight are waited on through that counter, in order).  The tables do not
+// change during a kernel.
 template <class TW>
-__device__ __forceinline__ void ld_root2_node1(TW tg, uint64_t &wu, uint64_t &wup, uint64_t &wv,
-                                               uint64_t &wvp) {
+__device__ __forceinline__ void ld_root2_u(TW tg, unsigned node, uint64_t &wu, uint64_t &wup,
+                                           uint64_t &wv, uint64_t &wvp) {
   typedef __attribute__((address_space(4))) const cp_u64x2 const_u64x2;
-  const const_u64x2 *k = (const const_u64x2 *)(uintptr_t)(tg + 4);
+  const const_u64x2 *k = (const const_u64x2 *)(uintptr_t)(tg + 4 * node);
   const cp_u64x2 a = k[0], b = k[1];
   wu = a.x;
   wup = a.y;
@@ -619,22 +621,22 @@ __device__ __forceinline__ void ld_root2_node1(TW tg, uint64_t &wu, uint64_t &wu
 }
 
 // One split-basis Gentleman-Sande butterfly: the group of A at global stage s
-// is pos >> (LOGN - s), its root at node 2^s + group.  At stage 0 the root is
-// the same for every butterfly (uniform, possibly in SGPRs).
+// is pos >> (LOGN - s), its root at node 2^s + group.  unode >= 0: the node,
+// the same for every butterfly of the stage (a uniform root, possibly in
+// SGPRs, which the compiler cannot see when pos comes from an opaque thread
+// index).
 template <int LOGN, class TW>
 __device__ __forceinline__ void gsi_bfly(cp_gi &A, cp_gi &B, int s, int pos, TW __restrict__ tg,
-                                         uint64_t p) {
-  // stage 0 has the single node 1 (pos < 2^LOGN): a uniform root, which the
-  // compiler cannot see when pos comes from an opaque thread index
+                                         uint64_t p, int unode) {
   uint64_t wu, wup, wv, wvp;
-  if (s == 0)
-    ld_root2_node1(tg, wu, wup, wv, wvp);
+  if (unode >= 0)
+    ld_root2_u(tg, (unsigned)unode, wu, wup, wv, wvp);
   else
     ld_root2(tg, (1u << s) + ((unsigned)pos >> (LOGN - s)), wu, wup, wv, wvp);
   const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
   A.re = cp_red2(A.re + B.re, p);
   A.imag = cp_red2(A.imag + B.imag, p);
-  if (s == 0)
+  if (unode >= 0)
     cp_shoup2(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
   else
     cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
@@ -657,7 +659,13 @@ __device__ __forceinline__ void gsi_pass(cp_gi (&x)[EPT], int t, TW __restrict__
 #pragma unroll
       for (int q = 0; q < U; q++) {
         if (q & half) continue;
-        gsi_bfly<LOGN>(x[ju * U + q], x[ju * U + q + half], s, map(base + q * ST), tg, p);
+        // stage 0 has the single node 1 (pos < 2^LOGN); with a block map the
+        // group of every slot is map(0) >> (LOGN - s) once LOGN - s >= LOGM
+        const int unode = s == 0 ? 1
+                          : (Map::kLocalMask && LOGN - s >= LOGM)
+                              ? (int)((1u << s) + ((unsigned)map(0) >> (LOGN - s)))
+                              : -1;
+        gsi_bfly<LOGN>(x[ju * U + q], x[ju * U + q + half], s, map(base + q * ST), tg, p, unode);
       }
       if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
@@ -825,8 +833,17 @@ __device__ __forceinline__ void park_ld(const CP_GLOBAL cp_gi *park, int area, i
   for (int i = 0; i < EPT; i++) x[i] = ld_gi(park + (area * EPT + i) * 1024 + t);
 }
 
+// src's half-0 slots of this thread (slot first_index<13,0,13>(t, i)), for
+// stage0_16k with PRE
+__device__ __forceinline__ void ld_half0_16k(cp_gi (&x)[EPT], const CP_GLOBAL cp_gi *src, int t) {
+  t = cp_opaque(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = ld_gi_nt(src + first_index<13, 0, 13>(t, i));
+}
+
 // conversion and twist (split basis) + forward stage 0 of src; half 0 -> x0,
-// half 1 -> x1
+// half 1 -> x1.  PRE: x0 already holds src's half-0 slots (ld_half0_16k).
+template <bool PRE = false>
 __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
                                            const CP_GLOBAL cp_gi *src, int t,
                                            const CP_GLOBAL uint64_t *twf,
@@ -837,7 +854,7 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
-    cp_gi x = ld_gi_nt(src + g), y = ld_gi_nt(src + g + H);
+    cp_gi x = PRE ? x0[i] : ld_gi_nt(src + g), y = ld_gi_nt(src + g + H);
     split_twist2(x, y, stw, g, g + H, J, Jp, p);
     uint64_t w, wp;
     ld_tw(twf, (unsigned)g, w, wp);
@@ -859,7 +876,7 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
   constexpr int H = 8192;
   t = cp_opaque(t);
   uint64_t wu, wup, wv, wvp;
-  ld_root2_node1(root, wu, wup, wv, wvp);
+  ld_root2_u(root, 1u, wu, wup, wv, wvp);
   const uint64_t J = sk[0], Jp = sk[1];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -876,8 +893,10 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
   }
 }
 
-// `park` must hold 3 * 8192 slots for this workgroup.  The halves run in
-// loops that are not unrolled, so each pass sequence is compiled once.
+// `park` must hold 3 * 8192 slots for this workgroup.  Every park read (and
+// the half of b's operand that fits) is loaded into the register set that
+// is free during the last pass of the preceding pass sequence, so its
+// latency overlaps that pass; the two sets swap roles from half to half.
 __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
                                                  const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
                                                  cp_gi *lds, const cp_prime &pr,
@@ -894,28 +913,28 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   // a: spectrum halves to P0, P1
   stage0_16k(x, y, a, t, twf, stw, J, Jp, p);
   park_st(park, 1, t, y);
-#pragma nounroll
-  for (int h = 0; h < 2; h++) {
-    if (h == 1) park_ld(park, 1, t, x);
-    dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{h * H});
-    park_st(park, h, t, x);
-  }
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0},
+                               [&] { park_ld(park, 1, t, y); });  // a's half 1 -> y
+  park_st(park, 0, t, x);
+  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H},
+                               [&] { ld_half0_16k(x, b, t); });  // b's half 0 -> x
+  park_st(park, 1, t, y);
   // b: per half, spectrum, product with a's (P0 / P1), inverse stages 13..1;
   // half 0's result waits in P0
-  stage0_16k(x, y, b, t, twf, stwh, J, Jp, p);
+  stage0_16k<true>(x, y, b, t, twf, stwh, J, Jp, p);
   park_st(park, 2, t, y);
-#pragma nounroll
-  for (int h = 0; h < 2; h++) {
-    if (h == 1) park_ld(park, 2, t, x);
-    dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{h * H});
-    park_ld(park, h, t, y);
+  dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0},
+                               [&] { park_ld(park, 0, t, y); });  // a's half 0 -> y
 #pragma unroll
-    for (int i = 0; i < EPT; i++) x[i] = split_mul(x[i], y[i], pr);
-    gsi_passes<14, 13, 0, 13, 1, true>(x, lds, t, root, p, MapBlock{h * H});
-    if (h == 0) park_st(park, 0, t, x);
-  }
+  for (int i = 0; i < EPT; i++) x[i] = split_mul(x[i], y[i], pr);
+  gsi_passes<14, 13, 0, 13, 1, true>(x, lds, t, root, p, MapBlock{0},
+                                     [&] { park_ld(park, 2, t, y); });  // b's half 1 -> y
+  park_st(park, 0, t, x);
+  dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H},
+                               [&] { park_ld(park, 1, t, x); });  // a's half 1 -> x
 #pragma unroll
-  for (int i = 0; i < EPT; i++) y[i] = x[i];
-  park_ld(park, 0, t, x);
+  for (int i = 0; i < EPT; i++) y[i] = split_mul(y[i], x[i], pr);
+  gsi_passes<14, 13, 0, 13, 1, true>(y, lds, t, root, p, MapBlock{H},
+                                     [&] { park_ld(park, 0, t, x); });  // half 0's result -> x
   istage0_16k(c, x, y, t, root, sk, p);
 }
