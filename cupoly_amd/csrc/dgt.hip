@@ -402,7 +402,8 @@ __global__ void __launch_bounds__((1 << LOGN) / EPT)
     const int rid = __builtin_amdgcn_readfirstlane(unit % nres);
     const cp_prime pr = primes[rid];
     const uint64_t p = pr.p;
-    const uint64_t *twf = twf_all + (size_t)rid * N;
+    const UTab twt = utab(uniform_ptr(twf_all + (size_t)rid * N), N * 16);
+    const UTab *twf = &twt;
     const CP_GLOBAL uint64_t *root =
         (const CP_GLOBAL uint64_t *)uniform_ptr(root_all + (size_t)rid * N * 4);
     const CP_GLOBAL uint64_t *stw =

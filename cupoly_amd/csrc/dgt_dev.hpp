@@ -66,6 +66,39 @@ __device__ __forceinline__ void ld_tw(const CP_GLOBAL uint64_t *__restrict__ tw,
   wp = v.y;
 }
 
+// A twiddle table whose base is wave-uniform, also held as a buffer
+// resource: a load whose index is a per-lane part plus a compile-time part
+// takes the first as a 32-bit VGPR offset and the second as an SGPR offset
+// (scalar adds), with no 64-bit address arithmetic per load.  Passed to the
+// pass functions by pointer (a local of the caller; it lives in SGPRs once
+// inlined).  Out-of-range buffer reads return 0: `bytes` is the table size.
+struct UTab {
+  __amdgpu_buffer_rsrc_t r;
+  const uint64_t *p;
+};
+__device__ __forceinline__ UTab utab(const uint64_t *p, unsigned bytes) {
+  // raw buffer, no stride: the gfx9 resource word 3
+  return UTab{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000), p};
+}
+// uniform index: the pointer path (a scalar load when the index is uniform)
+__device__ __forceinline__ void ld_tw(const UTab *tw, unsigned e, uint64_t &w, uint64_t &wp) {
+  ld_tw(tw->p, e, w, wp);
+}
+// index e_lane + e_const, e_const known at compile time
+__device__ __forceinline__ void ld_tw_lc(const UTab *tw, unsigned e_lane, unsigned e_const,
+                                         uint64_t &w, uint64_t &wp) {
+  const cp_u64x2 v = __builtin_bit_cast(
+      cp_u64x2, __builtin_amdgcn_raw_buffer_load_b128(tw->r, (int)(e_lane << 4),
+                                                      (int)(e_const << 4), 0));
+  w = v.x;
+  wp = v.y;
+}
+template <class P>
+__device__ __forceinline__ void ld_tw_lc(P tw, unsigned e_lane, unsigned e_const, uint64_t &w,
+                                         uint64_t &wp) {
+  ld_tw(tw, e_lane + e_const, w, wp);
+}
+
 // Wave-uniform pointer: kept in SGPRs (the compiler otherwise holds some
 // uniform bases in VGPR pairs across the transforms, where they spill).  The
 // rebuilt address is first typed as a global-memory pointer, so the accesses
@@ -116,11 +149,13 @@ __device__ __forceinline__ void dif_pass(cp_gi (&x)[EPT], int t, TW __restrict__
         } else {
           const int e = static_j ? (jc << s) : ((map(base + q * ST) & (m - 1)) << s);
           uint64_t w, wp;
-          ld_tw(tw, e, w, wp);
-          if (static_j)  // a uniform twiddle (may sit in SGPRs)
+          if (static_j) {  // a uniform twiddle (may sit in SGPRs)
+            ld_tw(tw, e, w, wp);
             cp_shoup2(dr, w, wp, di, w, wp, p, B.re, B.imag);
-          else
+          } else {
+            ld_tw_lc(tw, e, 0u, w, wp);
             cp_shoup2v(dr, w, wp, di, w, wp, p, B.re, B.imag);
+          }
         }
       }
       if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
@@ -284,11 +319,11 @@ __device__ __forceinline__ void dif_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
       const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
       A.re = cp_red2(sr, p);
       A.imag = cp_red2(si, p);
-      const unsigned e = Map::kLocalMask
-                             ? e0 + (unsigned)(q & (half - 1)) * (1u << (LOGM - LOG_EPT + SOFF + h))
-                             : (unsigned)((map(base + (q << lst)) & mmask) << s);
       uint64_t w, wp;
-      ld_tw(tw, e, w, wp);
+      if (Map::kLocalMask)
+        ld_tw_lc(tw, e0, (unsigned)(q & (half - 1)) * (1u << (LOGM - LOG_EPT + SOFF + h)), w, wp);
+      else
+        ld_tw(tw, (unsigned)((map(base + (q << lst)) & mmask) << s), w, wp);
       cp_shoup2v(dr, w, wp, di, w, wp, p, B.re, B.imag);
     }
     // one stage's twiddles in flight at a time (bounds VGPR use)
