@@ -141,19 +141,22 @@ __device__ __forceinline__ void uneven_small(const cp_unit *units, int unit0, in
     split_twist2(xa[i], xa[i + 1], stw, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
+  // b is loaded during a's last forward pass (as in the streaming product)
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id, [&] {
+    const int tq = cp_opaque(t);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) xb[i] = ld_gi_nt(ub + first_index<LOGN, 0, LOGN>(tq, i));
+  });
   const int tb = cp_opaque(t);
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     const int g0 = first_index<LOGN, 0, LOGN>(tb, i), g1 = first_index<LOGN, 0, LOGN>(tb, i + 1);
-    xb[i] = ld_gi_nt(ub + g0);
-    xb[i + 1] = ld_gi_nt(ub + g1);
     split_twist2(xb[i], xb[i + 1], stwh, g0, g1, J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
   dif_passes<LOGN, LOGN, 0, LOGN, 0>(xb, lds, t, twf, p, id);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) xa[i] = split_mul(xa[i], xb[i], pr);
+  for (int i = 0; i < EPT; i++) xa[i] = split_mul_paired(xa[i], xb[i], pr);
   gsi_passes<LOGN, LOGN, 0, LOGN, 0, true>(xa, lds, t, root, p, id);
   const int to = cp_opaque(t);
   if (active) {
