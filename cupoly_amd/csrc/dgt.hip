@@ -275,15 +275,18 @@ __global__ void __launch_bounds__(((1 << LOGN) / EPT) * ROWS, fused_min_waves<LO
                  first_index<LOGN, 0, LOGN>(t, i + 1), J, Jp, p);
     __builtin_amdgcn_sched_barrier(0);
   }
-  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id);
+  // b is loaded during a's last forward pass (its twiddles are scalar loads)
+  dif_passes<LOGN, LOGN, 0, LOGN, 0>(xa, lds, t, twf, p, id, [&] {
+    const int tq = cp_opaque(t);
+#pragma unroll
+    for (int i = 0; i < EPT; i++)
+      xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tq, i));
+  });
   // an opaque copy of the thread index: the address arithmetic of the later
   // loads and stores cannot be hoisted above this point (and held in VGPRs
   // across the transform).
   int tb = t;
   asm volatile("" : "+v"(tb));
-#pragma unroll
-  for (int i = 0; i < EPT; i++)
-    xb[i] = ld_gi_nt((const CP_GLOBAL cp_gi *)b + first_index<LOGN, 0, LOGN>(tb, i));
 #pragma unroll
   for (int i = 0; i < EPT; i += 2) {
     split_twist2(xb[i], xb[i + 1], stwh, first_index<LOGN, 0, LOGN>(tb, i),
