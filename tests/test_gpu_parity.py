@@ -383,13 +383,16 @@ def test_hom_mul_scale_forms_agree(gpu, k, n, batch):
                           O.hom_mul(P, cp.to_host(ct1[i]), cp.to_host(ct2[i])))
 
 
-@pytest.mark.parametrize("k,n,batch", [(2, 2048, 52), (3, 4096, 37), (3, 8192, 37),
-                                         (4, 16384, 29)])
+@pytest.mark.parametrize("k,n,batch", [(2, 2048, 52), (2, 2048, 53), (3, 4096, 37),
+                                         (3, 8192, 37), (4, 16384, 29)])
 def test_hom_mul_fused_and_composed_forms_agree(gpu, k, n, batch):
     """Batches with a (pair, residue) unit per CU (256 CUs: ceil(256 / (k + kl))
     pairs) run every transform of a unit in one workgroup; single pairs take
     the composed transforms.  Both forms give the same words for every pair,
-    and two pairs equal the oracle composition."""
+    and two pairs equal the oracle composition.  At n = 2^11 two units share
+    a workgroup: 53 pairs (265 units) leave the last workgroup's second row
+    idle, which still copies and transforms a clamped unit but stores
+    nothing."""
     import torch
     e, ctx = engine(k, k + 1, n, 65537)
     P = O.Params(k, k + 1, n, 65537)
