@@ -316,7 +316,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=20)
+    # the clock settles over the first ~40 launches of sustained load (launch
+    # times fall by ~20% from the third launch to the fortieth): 50 untimed
+    # launches (~60 ms) before the timed ones
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=4)
