@@ -348,7 +348,10 @@ __global__ void __launch_bounds__(1024)
 
 // Product for N = 16384 slots (n = 2^15): one 1024-thread workgroup per
 // residue product (polymul_16k_unit), persistent over the batch; each
-// workgroup owns 3 * 8192 park slots.
+// workgroup owns 3 * 8192 park slots.  Streaming: each unit starts loading
+// the next unit's a (half 0 into the LDS wave areas, half 1 into registers)
+// during its last inverse passes and its stores, so the workgroups' operand
+// reads overlap their arithmetic instead of arriving in one burst.
 __global__ void __launch_bounds__(1024)
     k_polymul_16k(cp_gi *__restrict__ out, const cp_gi *__restrict__ a,
                   const cp_gi *__restrict__ b, int nunits, int nres,
@@ -359,18 +362,28 @@ __global__ void __launch_bounds__(1024)
   constexpr int N = 16384;
   extern __shared__ cp_gi lds[];
   CP_GLOBAL cp_gi *park = (CP_GLOBAL cp_gi *)(park_all + (size_t)blockIdx.x * 3 * (N / 2));
-  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {  // uniform trip count per workgroup
+  const int t = (int)threadIdx.x;
+  int unit = blockIdx.x;
+  if (unit >= nunits) return;  // uniform
+  cp_gi x[EPT], y[EPT];
+  {  // the first unit's a: half 0 into the wave areas, half 1 into y
+    const CP_GLOBAL cp_gi *a0 = (const CP_GLOBAL cp_gi *)uniform_ptr(a + (size_t)unit * N);
+    lds_copy_issue<13>(lds + __builtin_amdgcn_readfirstlane(t >> 6) * (EPT * 64), a0, t);
+    ld_half1_16k(y, a0, t);
+  }
+  for (; unit < nunits; unit += gridDim.x) {  // uniform trip count per workgroup
     const int rid = unit % nres;
+    const int next = unit + (int)gridDim.x;
     const cp_prime pr = primes[rid];
     const size_t off = (size_t)unit * N;
-    polymul_16k_unit((CP_GLOBAL cp_gi *)(out + off), (const CP_GLOBAL cp_gi *)(a + off),
-                     (const CP_GLOBAL cp_gi *)(b + off), park, lds, pr,
-                     (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N),
-                     (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4),
-                     (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4),
-                     (const CP_GLOBAL uint64_t *)(stwh_all + (size_t)rid * N * 4),
-                     sk_all + (size_t)rid * 8);
-    __syncthreads();
+    polymul_16k_unit<true>(
+        (CP_GLOBAL cp_gi *)(out + off), (const CP_GLOBAL cp_gi *)(a + off),
+        (const CP_GLOBAL cp_gi *)uniform_ptr(b + off), park, lds, pr,
+        (const CP_GLOBAL uint64_t *)(twf_all + (size_t)rid * N),
+        (const CP_GLOBAL uint64_t *)(root_all + (size_t)rid * N * 4),
+        (const CP_GLOBAL uint64_t *)(stw_all + (size_t)rid * N * 4),
+        (const CP_GLOBAL uint64_t *)(stwh_all + (size_t)rid * N * 4), sk_all + (size_t)rid * 8, x,
+        y, (const CP_GLOBAL cp_gi *)uniform_ptr(a + (size_t)(next < nunits ? next : unit) * N));
   }
 }
 

@@ -708,9 +708,10 @@ __device__ __forceinline__ void gsi_pass(cp_gi (&x)[EPT], int t, TW __restrict__
 }
 
 // Full radix-8 split-basis inverse pass at a runtime local stage ls0.
+// uroot: load wave-uniform roots as scalars where the nodes allow it (below).
 template <int LOGN, int LOGM, int SOFF, class Map, class TW>
 __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restrict__ tg, uint64_t p,
-                                            const Map &map, int ls0) {
+                                            const Map &map, int ls0, bool uroot = false) {
   const int lst = LOGM - ls0 - LOG_EPT;
   const int base = ((t >> lst) << (lst + LOG_EPT)) + (t & ((1 << lst) - 1));
 #pragma unroll
@@ -724,24 +725,36 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
     // compile-time offset per butterfly
     const unsigned node0 =
         (1u << s) + ((unsigned)map(0) >> (LOGN - s)) + (((unsigned)t >> lst) << h);
+    // lst >= 6: t >> lst is the same for the 64 lanes of a wave, and so is
+    // every node of the stage: with uroot its roots are scalar loads
+    // (ld_root2_u), which leave the vector memory counter to loads a hook has
+    // put in flight
+    constexpr bool kBlock = Map::kLocalMask && LOGN == LOGM + SOFF;
+    const bool wave_node = kBlock && uroot && lst >= 6;
+    const unsigned node0u = wave_node ? __builtin_amdgcn_readfirstlane(node0) : node0;
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       if (q & half) continue;
       unsigned node;
-      if constexpr (Map::kLocalMask && LOGN == LOGM + SOFF) {
-        node = node0 + (unsigned)(q >> (LOG_EPT - h));
+      if constexpr (kBlock) {
+        node = node0u + (unsigned)(q >> (LOG_EPT - h));
       } else {
         const int pos = map(base + (q << lst));
         node = (1u << s) + ((unsigned)pos >> (LOGN - s));
       }
       uint64_t wu, wup, wv, wvp;
-      ld_root2(tg, node, wu, wup, wv, wvp);
       cp_gi &A = x[q];
       cp_gi &B = x[q + half];
       const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
       A.re = cp_red2(A.re + B.re, p);
       A.imag = cp_red2(A.imag + B.imag, p);
-      cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+      if (wave_node) {
+        ld_root2_u(tg, node, wu, wup, wv, wvp);
+        cp_shoup2(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+      } else {
+        ld_root2(tg, node, wu, wup, wv, wvp);
+        cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
+      }
     }
     if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
   }
@@ -753,12 +766,19 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
 // workgroup was the last exchange of dif_passes over the same stages, which
 // loaded the registers' layout with the map this sequence's first exchange
 // stores with, so the barrier before those stores can go as well.
-template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, bool AFTER_DIF = false, class Map,
-          class TW, class Hook = NoHook>
+// `hook` runs after the last LDS read (before the last pass); `early` runs
+// before the full pass of loop index EARLY_AT (counting down to 0, the last
+// full pass): register loads it starts are covered by that pass and the
+// ones after it, which must make no vector-memory loads of their own (wave-
+// uniform roots: gsi_pass_rt at lst >= 6).
+template <int LOGN, int LOGM, int LSB, int LSE, int SOFF, bool AFTER_DIF = false,
+          int EARLY_AT = -1, class Map, class TW, class Hook = NoHook, class Early = NoHook>
 __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            TW __restrict__ tg, uint64_t p, const Map &map,
-                                           const Hook &hook = Hook()) {
+                                           const Hook &hook = Hook(),
+                                           const Early &early = Early()) {
   using PP = PassPlan<LSB, LSE>;
+  static_assert(EARLY_AT < PP::NF - 1, "the early hook runs before a loop pass");
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
   t = cp_opaque(t);
   if constexpr (PP::NF > 0) gsi_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tg, p, map);
@@ -776,8 +796,9 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     lds_store_rt<LOGM>(lds, x, t, ls0 + LOG_EPT, m);
     if (local) wave_sync(); else __syncthreads();
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
+    if (i == EARLY_AT) early();
     if (PP::R0 == 0 && i == 0) hook();
-    gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0);
+    gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0, i <= EARLY_AT);
     prev_local = local;
     prev_pad = pad;
     prev_m = m;
@@ -876,9 +897,17 @@ __device__ __forceinline__ void ld_half0_16k(cp_gi (&x)[EPT], const CP_GLOBAL cp
   for (int i = 0; i < EPT; i++) x[i] = ld_gi_nt(src + first_index<13, 0, 13>(t, i));
 }
 
+// src's half-1 slots of this thread (slot first_index<13,0,13>(t, i) + 8192)
+__device__ __forceinline__ void ld_half1_16k(cp_gi (&x)[EPT], const CP_GLOBAL cp_gi *src, int t) {
+  t = cp_opaque(t);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) x[i] = ld_gi_nt(src + first_index<13, 0, 13>(t, i) + 8192);
+}
+
 // conversion and twist (split basis) + forward stage 0 of src; half 0 -> x0,
-// half 1 -> x1.  PRE: x0 already holds src's half-0 slots (ld_half0_16k).
-template <bool PRE = false>
+// half 1 -> x1.  PRE = 1: x0 already holds src's half-0 slots (ld_half0_16k);
+// PRE = 2: x0 and x1 hold both halves (nothing is loaded).
+template <int PRE = 0>
 __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
                                            const CP_GLOBAL cp_gi *src, int t,
                                            const CP_GLOBAL uint64_t *twf,
@@ -889,7 +918,7 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int g = first_index<13, 0, 13>(t, i);
-    cp_gi x = PRE ? x0[i] : ld_gi_nt(src + g), y = ld_gi_nt(src + g + H);
+    cp_gi x = PRE ? x0[i] : ld_gi_nt(src + g), y = PRE == 2 ? x1[i] : ld_gi_nt(src + g + H);
     split_twist2(x, y, stw, g, g + H, J, Jp, p);
     uint64_t w, wp;
     ld_tw(twf, (unsigned)g, w, wp);
@@ -903,11 +932,14 @@ __device__ __forceinline__ void stage0_16k(cp_gi (&x0)[EPT], cp_gi (&x1)[EPT],
 // split-tree inverse stage 0 (root r_0^-1 of node 1, the same for every
 // slot) of slots g and g + 8192 and the conversion back to coefficients
 // (the product's second operand carried h: split_back_h2), written to out;
-// sk = the residue's split constants
+// sk = the residue's split constants.  NEXT: once slot i is written, y[i]
+// takes slot i of nxt's half 1 (ld_half1_16k's layout).
+template <bool NEXT = false>
 __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&x)[EPT],
-                                            const cp_gi (&y)[EPT], int t,
+                                            cp_gi (&y)[EPT], int t,
                                             const CP_GLOBAL uint64_t *root,
-                                            const uint64_t *sk, uint64_t p) {
+                                            const uint64_t *sk, uint64_t p,
+                                            const CP_GLOBAL cp_gi *nxt = nullptr) {
   constexpr int H = 8192;
   t = cp_opaque(t);
   uint64_t wu, wup, wv, wvp;
@@ -924,14 +956,27 @@ __device__ __forceinline__ void istage0_16k(CP_GLOBAL cp_gi *out, const cp_gi (&
     split_back_h2(s, dd, J, Jp, p);
     st_gi_nt(out + g, s);
     st_gi_nt(out + g + H, dd);
+    if (NEXT) y[i] = ld_gi_nt(nxt + g + H);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// `park` must hold 3 * 8192 slots for this workgroup.  Every park read (and
-// the half of b's operand that fits) is loaded into the register set that
-// is free during the last pass of the preceding pass sequence, so its
-// latency overlaps that pass; the two sets swap roles from half to half.
+// `park` must hold 3 * 8192 slots for this workgroup.  Operand and park
+// reads overlap passes that make no vector-memory loads of their own:
+// - a forward sequence's last pass (uniform twiddles): a park read into the
+//   register set that is free, or b's half 0;
+// - an inverse sequence's last full pass and its last one-stage pass
+//   (wave-uniform roots): a park read.
+// (b's half 1 is read at b's stage 0: an LDS copy of it during a's last
+// forward pass, next to the register reads of b's half 0, is waited for as
+// soon as those registers are first moved, early in that pass.)
+// STREAM (one workgroup walking several units): on entry a's half 0 is being
+// copied into the wave areas and y holds a's half 1; before returning, the
+// unit starts the same for nxt, the next unit's a (the workgroup's last unit
+// passes its own a: the copy and loads are then unused, but unconditional,
+// so that the wait counts the compiler derives for the park reads do not
+// have to cover a path without them).
+template <bool STREAM = false>
 __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
                                                  const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
                                                  cp_gi *lds, const cp_prime &pr,
@@ -939,14 +984,22 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
                                                  const CP_GLOBAL uint64_t *root,
                                                  const CP_GLOBAL uint64_t *stw,
                                                  const CP_GLOBAL uint64_t *stwh,
-                                                 const uint64_t *sk) {
+                                                 const uint64_t *sk, cp_gi (&x)[EPT],
+                                                 cp_gi (&y)[EPT],
+                                                 const CP_GLOBAL cp_gi *nxt = nullptr) {
   constexpr int H = 8192;
-  const int t = (int)threadIdx.x;
+  const int t = (int)threadIdx.x, lane = t & 63;
+  cp_gi *area = lds + __builtin_amdgcn_readfirstlane(t >> 6) * (EPT * 64);
   const uint64_t p = pr.p;
   const uint64_t J = sk[0], Jp = sk[1];
-  cp_gi x[EPT], y[EPT];
   // a: spectrum halves to P0, P1
-  stage0_16k(x, y, a, t, twf, stw, J, Jp, p);
+  if constexpr (STREAM) {
+    lds_copy_wait();
+    lds_copy_read(x, area, lane);
+    stage0_16k<2>(x, y, a, t, twf, stw, J, Jp, p);
+  } else {
+    stage0_16k(x, y, a, t, twf, stw, J, Jp, p);
+  }
   park_st(park, 1, t, y);
   dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0},
                                [&] { park_ld(park, 1, t, y); });  // a's half 1 -> y
@@ -956,20 +1009,43 @@ __device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GL
   park_st(park, 1, t, y);
   // b: per half, spectrum, product with a's (P0 / P1), inverse stages 13..1;
   // half 0's result waits in P0
-  stage0_16k<true>(x, y, b, t, twf, stwh, J, Jp, p);
+  stage0_16k<1>(x, y, b, t, twf, stwh, J, Jp, p);
   park_st(park, 2, t, y);
   dif_passes<14, 13, 0, 13, 1>(x, lds, t, twf, p, MapBlock{0},
                                [&] { park_ld(park, 0, t, y); });  // a's half 0 -> y
 #pragma unroll
   for (int i = 0; i < EPT; i++) x[i] = split_mul(x[i], y[i], pr);
-  gsi_passes<14, 13, 0, 13, 1, true>(x, lds, t, root, p, MapBlock{0},
-                                     [&] { park_ld(park, 2, t, y); });  // b's half 1 -> y
+  gsi_passes<14, 13, 0, 13, 1, true, 0>(x, lds, t, root, p, MapBlock{0}, NoHook(),
+                                        [&] { park_ld(park, 2, t, y); });  // b's half 1 -> y
   park_st(park, 0, t, x);
   dif_passes<14, 13, 0, 13, 1>(y, lds, t, twf, p, MapBlock{H},
                                [&] { park_ld(park, 1, t, x); });  // a's half 1 -> x
 #pragma unroll
   for (int i = 0; i < EPT; i++) y[i] = split_mul(y[i], x[i], pr);
-  gsi_passes<14, 13, 0, 13, 1, true>(y, lds, t, root, p, MapBlock{H},
-                                     [&] { park_ld(park, 0, t, x); });  // half 0's result -> x
-  istage0_16k(c, x, y, t, root, sk, p);
+  gsi_passes<14, 13, 0, 13, 1, true, 0>(
+      y, lds, t, root, p, MapBlock{H},
+      [&] {
+        if constexpr (STREAM) {  // the next a's half 0 -> the wave areas
+          // x's park reads (a full pass ago) are waited for here, before the
+          // copy: a wait for a load issued ahead of an LDS copy is compiled
+          // as a wait for the copy too
+          pin_regs(x);
+          __syncthreads();
+          lds_copy_issue<13>(area, nxt, t);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      },
+      [&] { park_ld(park, 0, t, x); });  // half 0's result -> x
+  istage0_16k<STREAM>(c, x, y, t, root, sk, p, nxt);  // STREAM: the next a's half 1 -> y
+}
+__device__ __forceinline__ void polymul_16k_unit(CP_GLOBAL cp_gi *c, const CP_GLOBAL cp_gi *a,
+                                                 const CP_GLOBAL cp_gi *b, CP_GLOBAL cp_gi *park,
+                                                 cp_gi *lds, const cp_prime &pr,
+                                                 const CP_GLOBAL uint64_t *twf,
+                                                 const CP_GLOBAL uint64_t *root,
+                                                 const CP_GLOBAL uint64_t *stw,
+                                                 const CP_GLOBAL uint64_t *stwh,
+                                                 const uint64_t *sk) {
+  cp_gi x[EPT], y[EPT];
+  polymul_16k_unit<false>(c, a, b, park, lds, pr, twf, root, stw, stwh, sk, x, y);
 }
