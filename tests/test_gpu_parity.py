@@ -526,13 +526,14 @@ def test_polymul_persistent_choice_needs_n_2048_or_more(gpu):
         cp.polymul_set_tuning(ctx, 0, 1)
 
 
-@pytest.mark.parametrize("n,k,batch", [(16384, 4, 100), (4096, 3, 400)])
+@pytest.mark.parametrize("n,k,batch", [(16384, 4, 100), (4096, 3, 400), (32768, 3, 230)])
 def test_polymul_streaming_many_units_per_workgroup(gpu, n, k, batch):
     """More residue products than the streaming grid holds, so each
     workgroup runs several units, each with its operands copied into LDS
-    during the previous transform's last pass: equal to the one-shot form for
-    every product (not a multiple of the grid), and to the oracle for the first
-    and last."""
+    during the previous transform's last pass (at n = 2^15 the next unit's a
+    read during the last inverse pass, 690 units: two or three per
+    workgroup): equal to the one-shot (n = 2^15: split) form for every product
+    (not a multiple of the grid), and to the oracle for the first and last."""
     e, ctx = engine(k, k + 1, n)
     P = O.Params(k, k + 1, n, 65537)
     a = rand_res(P, P.q, (batch,), 900 + n)
