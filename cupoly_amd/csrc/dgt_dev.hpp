@@ -778,7 +778,7 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
                                            const Hook &hook = Hook(),
                                            const Early &early = Early()) {
   using PP = PassPlan<LSB, LSE>;
-  static_assert(EARLY_AT < PP::NF - 1, "the early hook runs before a loop pass");
+  static_assert(EARLY_AT < 0 || EARLY_AT < PP::NF - 1, "the early hook runs before a loop pass");
   constexpr LdsMap MP = lds_map_for(LOGM - PP::LSF - LOG_EPT);
   t = cp_opaque(t);
   if constexpr (PP::NF > 0) gsi_pass<LOGN, LOGM, PP::LSL, LOG_EPT, SOFF>(x, t, tg, p, map);
