@@ -708,8 +708,9 @@ __device__ __forceinline__ void gsi_pass(cp_gi (&x)[EPT], int t, TW __restrict__
 }
 
 // Full radix-8 split-basis inverse pass at a runtime local stage ls0.
-// uroot: load wave-uniform roots as scalars where the nodes allow it (below).
-template <int LOGN, int LOGM, int SOFF, class Map, class TW>
+// UROOT and uroot: load wave-uniform roots as scalars where the nodes allow
+// it (below); without UROOT the pass compiles as if that option did not exist.
+template <int LOGN, int LOGM, int SOFF, bool UROOT = false, class Map, class TW>
 __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restrict__ tg, uint64_t p,
                                             const Map &map, int ls0, bool uroot = false) {
   const int lst = LOGM - ls0 - LOG_EPT;
@@ -730,7 +731,7 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
     // (ld_root2_u), which leave the vector memory counter to loads a hook has
     // put in flight
     constexpr bool kBlock = Map::kLocalMask && LOGN == LOGM + SOFF;
-    const bool wave_node = kBlock && uroot && lst >= 6;
+    const bool wave_node = UROOT && kBlock && uroot && lst >= 6;
     const unsigned node0u = wave_node ? __builtin_amdgcn_readfirstlane(node0) : node0;
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
@@ -743,18 +744,19 @@ __device__ __forceinline__ void gsi_pass_rt(cp_gi (&x)[EPT], int t, TW __restric
         node = (1u << s) + ((unsigned)pos >> (LOGN - s));
       }
       uint64_t wu, wup, wv, wvp;
+      if (wave_node)
+        ld_root2_u(tg, node, wu, wup, wv, wvp);
+      else
+        ld_root2(tg, node, wu, wup, wv, wvp);
       cp_gi &A = x[q];
       cp_gi &B = x[q + half];
       const uint64_t dr = A.re + p - B.re, di = A.imag + p - B.imag;
       A.re = cp_red2(A.re + B.re, p);
       A.imag = cp_red2(A.imag + B.imag, p);
-      if (wave_node) {
-        ld_root2_u(tg, node, wu, wup, wv, wvp);
+      if (wave_node)
         cp_shoup2(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
-      } else {
-        ld_root2(tg, node, wu, wup, wv, wvp);
+      else
         cp_shoup2v(dr, wu, wup, di, wv, wvp, p, B.re, B.imag);
-      }
     }
     if (CP_STAGE_BARRIER) __builtin_amdgcn_sched_barrier(0);
   }
@@ -798,7 +800,7 @@ __device__ __forceinline__ void gsi_passes(cp_gi (&x)[EPT], cp_gi *lds, int t,
     lds_load_rt<LOGM>(lds, x, t, ls0, m);
     if (i == EARLY_AT) early();
     if (PP::R0 == 0 && i == 0) hook();
-    gsi_pass_rt<LOGN, LOGM, SOFF>(x, t, tg, p, map, ls0, i <= EARLY_AT);
+    gsi_pass_rt<LOGN, LOGM, SOFF, (EARLY_AT >= 0)>(x, t, tg, p, map, ls0, i <= EARLY_AT);
     prev_local = local;
     prev_pad = pad;
     prev_m = m;
