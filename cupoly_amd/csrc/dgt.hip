@@ -167,6 +167,100 @@ __global__ void __launch_bounds__((1 << LOGM_) / EPT)
   (void)nunits;
 }
 
+// One 16384-slot residue polynomial (n = 2^15) per 1024-thread workgroup, in
+// place, in the standard layout: the values of the split launch (k_dgt_cols +
+// k_dgt_blocks; the same network, the arithmetic exact) with one read and one
+// write of the polynomial and one thread-major park round trip of a half.
+// Stage 0 pairs slot g with g + 8192 in registers (g = first_index<13,0,13>
+// (t, i): the thread's slots are t modulo 1024), and the two 8192-slot halves
+// take stages 1..13 in turn through the LDS, each stored (forward) or loaded
+// (inverse) in natural order through the LDS.  The parked half waits in the
+// polynomial itself:
+// - forward: half 1 at H + i * 1024 + t, slots this thread has read;
+// - inverse: half 0 at i * 1024 + t, after every thread has loaded half 0
+//   into the LDS (the barriers of its passes come between).
+template <bool FWD>
+__global__ void __launch_bounds__(1024)
+    k_dgt_single16(cp_gi *__restrict__ data, int nres, int roff,
+                   const cp_prime *__restrict__ primes, const uint64_t *__restrict__ tw_all,
+                   const uint64_t *__restrict__ twist_all) {
+  constexpr int N = 16384, H = 8192, LOGM = 13;
+  using LP = LastPass<LOGM, 0, LOGM>;
+  extern __shared__ cp_gi lds[];
+  const int t = (int)threadIdx.x;
+  const int unit = (int)blockIdx.x;
+  const int rid = __builtin_amdgcn_readfirstlane(roff + unit % nres);
+  const uint64_t p = primes[rid].p;
+  const uint64_t *tw = tw_all + (size_t)rid * N;
+  const uint64_t *tt = twist_all + (size_t)rid * N * 6;
+  CP_GLOBAL cp_gi *gp = (CP_GLOBAL cp_gi *)(data + (size_t)unit * N);
+  cp_gi x[EPT], y[EPT];
+  if constexpr (FWD) {
+    {
+      const int ta = cp_opaque(t);
+#pragma unroll
+      for (int i = 0; i < EPT; i++) {  // twist, then global stage 0 (twiddle omega^g)
+        const int g = first_index<LOGM, 0, LOGM>(ta, i);
+        cp_gi a = ld_gi(gp + g), b = ld_gi(gp + g + H);
+        twist(a, tt, g, p);
+        twist(b, tt, g + H, p);
+        uint64_t w, wp;
+        ld_tw(tw, (unsigned)g, w, wp);
+        x[i].re = cp_red2(a.re + b.re, p);
+        x[i].imag = cp_red2(a.imag + b.imag, p);
+        cp_shoup2v(a.re + p - b.re, w, wp, a.imag + p - b.imag, w, wp, p, y[i].re, y[i].imag);
+      }
+    }
+    park_st(gp + H, 0, t, y);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (h == 1) park_ld(gp + H, 0, t, x);
+      dif_passes<14, LOGM, 0, LOGM, 1>(x, lds, t, tw, p, MapBlock{h * H});
+      __syncthreads();
+      lds_store<LOGM, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < EPT; i++)
+        st_gi(gp + h * H + nat_index<LOGM>(t, i), lds[lds_phys(nat_index<LOGM>(t, i), LP::MS)]);
+    }
+  } else {
+    // both halves pass through x (half 0 then parked, read back into y)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (h == 1) __syncthreads();  // the LDS reads of half 0's passes are done
+#pragma unroll
+      for (int i = 0; i < EPT; i++)
+        lds[lds_phys(nat_index<LOGM>(t, i), LP::MS)] = ld_gi(gp + h * H + nat_index<LOGM>(t, i));
+      __syncthreads();
+      lds_load<LOGM, LP::LS0, LP::RL, LP::MS.A, LP::MS.S>(lds, x, t);
+      dit_passes<14, LOGM, 0, LOGM, 1>(x, lds, t, tw, p, MapBlock{h * H});
+      if (h == 0) park_st(gp, 0, t, x);
+    }
+    // no global store comes between the park's store and this read, so the
+    // compiler could otherwise hoist the read above half 1's passes
+    asm volatile("" ::: "memory");
+    park_ld(gp, 0, t, y);
+    const int ta = cp_opaque(t);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {  // global stage 0 (twiddle omega^-g), then the untwist
+      const int g = first_index<LOGM, 0, LOGM>(ta, i);
+      uint64_t w, wp, tr, ti;
+      ld_tw(tw, (unsigned)g, w, wp);
+      cp_shoup2v(x[i].re, w, wp, x[i].imag, w, wp, p, tr, ti);  // half 1 (slot g + H)
+      cp_gi a, b;
+      b.re = cp_red2(y[i].re + p - tr, p);
+      b.imag = cp_red2(y[i].imag + p - ti, p);
+      a.re = cp_red2(y[i].re + tr, p);
+      a.imag = cp_red2(y[i].imag + ti, p);
+      twist(a, tt, g, p);
+      twist(b, tt, g + H, p);
+      st_gi(gp + g, a);
+      st_gi(gp + g + H, b);
+      __builtin_amdgcn_sched_barrier(0);  // one slot's table reads in flight at a time
+    }
+  }
+}
+
 // Block stages [S, LOGN): each workgroup row is one 128-slot block.
 template <int LOGN, bool FWD, int LOGM_ = 13>
 __global__ void __launch_bounds__((1 << LOGM_) / EPT)
@@ -519,6 +613,10 @@ static int setup_size() {
     if ((rc = set_lds(k_dgt_blocks<LOGN, true>, lds))) return rc;
     if ((rc = set_lds(k_dgt_blocks<LOGN, false>, lds))) return rc;
     if ((rc = set_lds(k_polymul_blocks<LOGN>, lds))) return rc;
+    if constexpr (LOGN == 14) {
+      if ((rc = set_lds(k_dgt_single16<true>, lds))) return rc;
+      if ((rc = set_lds(k_dgt_single16<false>, lds))) return rc;
+    }
     if constexpr (LOGN == 14)
       if ((rc = set_lds(k_polymul_16k, lds))) return rc;
   }
@@ -566,8 +664,10 @@ static void launch_split(cupoly_ctx *c, cp_gi *data, int roff, int nres, int nun
 }
 
 // One transform launch form per call: N <= 8192 slots in one workgroup per
-// residue polynomial (k_dgt_single), N = 16384, 32768 in the 1024-thread
-// split; when that would give fewer workgroups than CUs and N >= 1024, the
+// residue polynomial (k_dgt_single); N = 16384 in one workgroup per residue
+// polynomial as well (k_dgt_single16) once those cover 5/8 of the CUs;
+// otherwise N = 16384, 32768 in the 1024-thread split; when that would give
+// fewer workgroups than CUs and N >= 1024, the
 // 64-thread split, which spreads each residue polynomial over 2^(LOGN-9)
 // workgroups per kernel (the reference's two-pass form, dgt.cu:125-160,
 // whose passes run on Na * nres blocks).
@@ -595,6 +695,15 @@ static int dgt_launch_t(cupoly_ctx *c, cp_gi *data, int roff, int nres, int npol
     else
       hipLaunchKernelGGL((k_dgt_single<LOGN, R, false>), dim3(grid), dim3(T * R), lds, c->stream,
                          data, nunits, nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
+  } else if (LOGN == 14 && 8 * nunits >= 5 * e->ncu) {
+    // one workgroup per residue polynomial once they cover 5/8 of the CUs
+    const size_t lds = (size_t)lds_row<13>() * sizeof(cp_gi);
+    if (dir == CUPOLY_FORWARD)
+      hipLaunchKernelGGL(k_dgt_single16<true>, dim3(nunits), dim3(1024), lds, c->stream, data, nres,
+                         roff, e->d_primes, e->d_tw_fwd, e->d_twist_fwd);
+    else
+      hipLaunchKernelGGL(k_dgt_single16<false>, dim3(nunits), dim3(1024), lds, c->stream, data,
+                         nres, roff, e->d_primes, e->d_tw_inv, e->d_twist_inv);
   } else {
     launch_split<LOGN, 13>(c, data, roff, nres, nunits, dir);
   }

@@ -71,6 +71,31 @@ def test_relinearize_matches_oracle(gpu, k, n, batch):
     assert torch.equal(dy, cp.to_device(y))
 
 
+def test_relinearize_n32768_one_workgroup_transforms(gpu):
+    """n = 2^15, k = 8, 12 pairs: the digits' forward transforms (768 residue
+    polynomials) and the accumulators' inverse transforms (192) cover 5/8 of
+    the CUs, so they run one workgroup per residue polynomial
+    (k_dgt_single16); the pairs relinearized one at a time take the split
+    transforms.  Equal for every pair, and equal to the oracle for two."""
+    k, n, batch = 8, 32768, 12
+    e, ctx, P, s_vals, rk = setup(k, n)
+    rng = np.random.default_rng(9)
+    y = np.empty((batch, 3, k, P.Ng, 2), dtype=np.uint64)
+    for j, qj in enumerate(P.q):
+        y[:, :, j] = rng.integers(0, qj, size=(batch, 3, P.Ng, 2), dtype=np.uint64)
+    dy = cp.to_device(y)
+    out = cp.empty((batch, 2, k, P.Ng, 2))
+    cp.relinearize(ctx, rk, out, dy)
+    h = cp.to_host(out)
+    one = cp.empty((1, 2, k, P.Ng, 2))
+    for b in range(batch):
+        cp.relinearize(ctx, rk, one, dy[b:b + 1])
+        assert np.array_equal(cp.to_host(one)[0], h[b]), b
+    key = key_on_host(ctx, rk, P)
+    for b in (0, batch - 1):
+        assert np.array_equal(h[b], O.relinearize(P, y[b], key)), b
+
+
 @pytest.mark.parametrize("k,n,batch", [(3, 8192, 4), (4, 16384, 2), (8, 32768, 2), (2, 2048, 24)])
 def test_hom_mul_relin_equals_composition(gpu, k, n, batch):
     """hom_mul_relin_batch = relinearize(hom_mul_batch), and the gathered form
